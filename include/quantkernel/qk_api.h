@@ -1,0 +1,215 @@
+/*
+ * qk_api.h - flat C ABI of the MI355X Monte Carlo batch-pricing library.
+ *
+ * Every declaration marked "reference" has the exact name, argument order,
+ * types and return convention of the reference's declaration at the cited
+ * line of cpp/include/quantkernel/qk_api.h, so the reference's callers
+ * (python/quantkernel/_loader.py:152-199, engine.py:163-194, C tests) bind to
+ * this library unchanged.  Semantics kept from the reference
+ * (cpp/src/qk_api.cpp:19-117, 837-1227):
+ *   - batch calls null-check every pointer in declaration order first
+ *     ("null pointer: <param>", QK_ERR_NULL_PTR), then n <= 0
+ *     ("bad batch size: <n>", QK_ERR_BAD_SIZE);
+ *   - an invalid element writes a quiet NaN and the call still returns QK_OK;
+ *   - the last-error string is thread-local, 256 bytes, sticky on success and
+ *     cleared only by qk_clear_last_error.
+ * What changes: the paths are simulated on a gfx950 GPU.  A missing or
+ * unusable device makes a batch call return QK_ERR_RUNTIME_INIT (there is no
+ * CPU fallback) and a scalar call return NaN with the error string set.
+ *
+ * All pointers of the non-"_device" entry points are caller-owned host memory
+ * of length n.  Calls are synchronous and may come from several host threads
+ * (device work is serialised internally).
+ */
+#ifndef QK_API_H
+#define QK_API_H
+
+#include "qk_abi.h"
+
+#define QK_EXPORT __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- ABI negotiation and errors (reference qk_api.h:16-21) ------------- */
+QK_EXPORT void        qk_abi_version(int32_t* major, int32_t* minor);
+QK_EXPORT int32_t     qk_plugin_get_api(int32_t host_abi_major, int32_t host_abi_minor,
+                                        const QKPluginAPI** out_api);
+QK_EXPORT const char* qk_get_last_error(void);
+QK_EXPORT void        qk_clear_last_error(void);
+
+/* ---- GBM terminal, antithetic pairs (reference qk_api.h:259-268) -------- */
+QK_EXPORT double  qk_mcm_standard_monte_carlo_price(double spot, double strike, double t, double vol,
+                                                    double r, double q, int32_t option_type,
+                                                    int32_t paths, uint64_t seed);
+QK_EXPORT int32_t qk_mcm_standard_monte_carlo_price_batch(const double* spot, const double* strike,
+                                                          const double* t, const double* vol,
+                                                          const double* r, const double* q,
+                                                          const int32_t* option_type,
+                                                          const int32_t* paths,
+                                                          const uint64_t* seed,
+                                                          int32_t n, double* out_prices);
+
+/* ---- antithetic variates (reference qk_api.h:294-296, 347-352) ---------- */
+QK_EXPORT double  qk_mcm_antithetic_variates_price(double spot, double strike, double t, double vol,
+                                                   double r, double q, int32_t option_type,
+                                                   int32_t paths, uint64_t seed);
+QK_EXPORT int32_t qk_mcm_antithetic_variates_price_batch(const double* spot, const double* strike,
+                                                         const double* t, const double* vol,
+                                                         const double* r, const double* q,
+                                                         const int32_t* option_type, const int32_t* paths,
+                                                         const uint64_t* seed,
+                                                         int32_t n, double* out_prices);
+
+/* ---- GBM Euler-Maruyama (reference qk_api.h:269-271, 300-305) ----------- */
+QK_EXPORT double  qk_mcm_euler_maruyama_price(double spot, double strike, double t, double vol,
+                                              double r, double q, int32_t option_type,
+                                              int32_t paths, int32_t steps, uint64_t seed);
+QK_EXPORT int32_t qk_mcm_euler_maruyama_price_batch(const double* spot, const double* strike,
+                                                    const double* t, const double* vol,
+                                                    const double* r, const double* q,
+                                                    const int32_t* option_type, const int32_t* paths,
+                                                    const int32_t* steps, const uint64_t* seed,
+                                                    int32_t n, double* out_prices);
+
+/* ---- GBM Milstein (reference qk_api.h:272-274, 306-311) ----------------- */
+QK_EXPORT double  qk_mcm_milstein_price(double spot, double strike, double t, double vol,
+                                        double r, double q, int32_t option_type,
+                                        int32_t paths, int32_t steps, uint64_t seed);
+QK_EXPORT int32_t qk_mcm_milstein_price_batch(const double* spot, const double* strike,
+                                              const double* t, const double* vol,
+                                              const double* r, const double* q,
+                                              const int32_t* option_type, const int32_t* paths,
+                                              const int32_t* steps, const uint64_t* seed,
+                                              int32_t n, double* out_prices);
+
+/* ---- Longstaff-Schwartz American (reference qk_api.h:275-277, 312-317) -- */
+QK_EXPORT double  qk_mcm_longstaff_schwartz_price(double spot, double strike, double t, double vol,
+                                                  double r, double q, int32_t option_type,
+                                                  int32_t paths, int32_t steps, uint64_t seed);
+QK_EXPORT int32_t qk_mcm_longstaff_schwartz_price_batch(const double* spot, const double* strike,
+                                                        const double* t, const double* vol,
+                                                        const double* r, const double* q,
+                                                        const int32_t* option_type, const int32_t* paths,
+                                                        const int32_t* steps, const uint64_t* seed,
+                                                        int32_t n, double* out_prices);
+
+/* ---- Heston full-truncation Euler (reference qk_api.h:359-374) ---------- */
+QK_EXPORT double  qk_mcm_heston_monte_carlo_price(double spot, double strike, double t,
+                                                  double r, double q,
+                                                  double v0, double kappa, double theta,
+                                                  double sigma, double rho,
+                                                  int32_t option_type,
+                                                  int32_t paths, int32_t steps, uint64_t seed);
+QK_EXPORT int32_t qk_mcm_heston_monte_carlo_price_batch(const double* spot, const double* strike,
+                                                        const double* t,
+                                                        const double* r, const double* q,
+                                                        const double* v0, const double* kappa,
+                                                        const double* theta, const double* sigma,
+                                                        const double* rho,
+                                                        const int32_t* option_type,
+                                                        const int32_t* paths, const int32_t* steps,
+                                                        const uint64_t* seed,
+                                                        int32_t n, double* out_prices);
+
+/* ---- constant local vol, Euler (reference qk_api.h:393-403) ------------- */
+QK_EXPORT double  qk_mcm_local_vol_monte_carlo_price(double spot, double strike, double t,
+                                                     double vol, double r, double q,
+                                                     int32_t option_type,
+                                                     int32_t paths, int32_t steps, uint64_t seed);
+QK_EXPORT int32_t qk_mcm_local_vol_monte_carlo_price_batch(const double* spot, const double* strike,
+                                                           const double* t, const double* vol,
+                                                           const double* r, const double* q,
+                                                           const int32_t* option_type,
+                                                           const int32_t* paths, const int32_t* steps,
+                                                           const uint64_t* seed,
+                                                           int32_t n, double* out_prices);
+
+/* ======================================================================== *
+ * ABI 2.13 additions.  No reference counterpart: new names only.
+ * ======================================================================== */
+
+/* Process-wide normal-stream mode (QK_RNG_*) used by every MC entry point.
+ * Returns QK_ERR_INVALID_INPUT for an unknown mode.  Default QK_RNG_PHILOX. */
+QK_EXPORT int32_t qk_mc_set_rng_mode(int32_t mode);
+QK_EXPORT int32_t qk_mc_get_rng_mode(void);
+
+/* Batch entry points that also return the standard error of each price
+ * (discounted sample standard deviation / sqrt(paths); for antithetic
+ * estimators the pair average is the sample unit).  Same argument order as the
+ * reference batch plus out_stderr; out_stderr is 0 for closed-form branches
+ * and NaN for invalid elements. */
+QK_EXPORT int32_t qk_mcm_standard_monte_carlo_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const uint64_t* seed,
+        int32_t n, double* out_prices, double* out_stderr);
+QK_EXPORT int32_t qk_mcm_euler_maruyama_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const int32_t* steps, const uint64_t* seed,
+        int32_t n, double* out_prices, double* out_stderr);
+QK_EXPORT int32_t qk_mcm_milstein_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const int32_t* steps, const uint64_t* seed,
+        int32_t n, double* out_prices, double* out_stderr);
+QK_EXPORT int32_t qk_mcm_local_vol_monte_carlo_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const int32_t* steps, const uint64_t* seed,
+        int32_t n, double* out_prices, double* out_stderr);
+QK_EXPORT int32_t qk_mcm_heston_monte_carlo_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* r, const double* q,
+        const double* v0, const double* kappa, const double* theta, const double* sigma,
+        const double* rho, const int32_t* option_type, const int32_t* paths, const int32_t* steps,
+        const uint64_t* seed, int32_t n, double* out_prices, double* out_stderr);
+
+/* Mixed-model uneven batch: one persistent launch over options of different
+ * models (QK_MODEL_*), path counts and step counts.  vol is used by the GBM
+ * and local-vol models, v0..rho by Heston; unused arrays may hold anything
+ * but must not be null.  Per element the result equals the single-model batch
+ * entry point's result for the same inputs. */
+QK_EXPORT int32_t qk_mcm_mixed_price_batch(const int32_t* model,
+        const double* spot, const double* strike, const double* t, const double* vol,
+        const double* r, const double* q,
+        const double* v0, const double* kappa, const double* theta, const double* sigma,
+        const double* rho, const int32_t* option_type, const int32_t* paths, const int32_t* steps,
+        const uint64_t* seed, int32_t n, double* out_prices, double* out_stderr);
+
+/* Same as qk_mcm_mixed_price_batch, but every pointer is device memory of
+ * the current HIP device (inputs already resident in HBM, outputs left there).
+ * QK_RNG_PHILOX only; returns QK_ERR_INVALID_INPUT in another RNG mode. */
+QK_EXPORT int32_t qk_mcm_mixed_price_batch_device(const int32_t* model,
+        const double* spot, const double* strike, const double* t, const double* vol,
+        const double* r, const double* q,
+        const double* v0, const double* kappa, const double* theta, const double* sigma,
+        const double* rho, const int32_t* option_type, const int32_t* paths, const int32_t* steps,
+        const uint64_t* seed, int32_t n, double* out_prices, double* out_stderr);
+
+/* Longstaff-Schwartz with a selectable regression basis (QK_LSMC_BASIS_*)
+ * and standard-error output.  basis QK_LSMC_BASIS_REFERENCE reproduces
+ * qk_mcm_longstaff_schwartz_price_batch. */
+QK_EXPORT int32_t qk_mcm_longstaff_schwartz_price_batch_ex(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const int32_t* steps,
+        const uint64_t* seed, int32_t basis, int32_t n, double* out_prices, double* out_stderr);
+
+/* Device timings of the last MC call made by this process, from HIP events on
+ * the library's stream: total device time of the call and the time of its
+ * path-simulation kernel alone, in milliseconds.  Returns QK_ERR_RUNTIME_INIT
+ * if no call has completed yet. */
+QK_EXPORT int32_t qk_mc_last_timing(double* ms_total, double* ms_simulate);
+
+/* The normal stream the QK_RNG_MT19937_* modes feed to the GPU for one
+ * option seed: the first `count` draws of std::mt19937_64(seed) through
+ * std::normal_distribution<double> (reference mc_engine.h:15-23).  Host
+ * only; needs no device. */
+QK_EXPORT int32_t qk_mc_validation_stream(uint64_t seed, int64_t count, double* out);
+
+/* Name of the device in use and its compute-unit count; QK_ERR_RUNTIME_INIT
+ * if there is none. */
+QK_EXPORT int32_t qk_device_info(char* name_buf, int32_t buf_len, int32_t* compute_units);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QK_API_H */

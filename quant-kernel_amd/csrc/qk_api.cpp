@@ -1,0 +1,465 @@
+// qk_api.cpp - flat C ABI of the MI355X Monte Carlo batch-pricing library.
+//
+// Error conventions follow the reference's qk_api.cpp:19-117:
+//   - a thread-local 256-byte message, strncpy-truncated, sticky on success,
+//     cleared only by qk_clear_last_error;
+//   - batch calls check every pointer in declaration order first
+//     ("null pointer: <param>" -> QK_ERR_NULL_PTR), then n <= 0
+//     ("bad batch size: <n>" -> QK_ERR_BAD_SIZE);
+//   - invalid elements give NaN with QK_OK.
+// Scalar entry points are batches of one, so scalar == batch bitwise.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "qk_engine.hpp"
+#include "qk_lsmc.hpp"
+#include "quantkernel/qk_api.h"
+
+namespace {
+
+thread_local char tl_msg[256] = "";
+
+void set_msg(const char* m) {
+    std::strncpy(tl_msg, m, sizeof(tl_msg) - 1);
+    tl_msg[sizeof(tl_msg) - 1] = '\0';
+}
+
+void set_msg(const std::string& m) { set_msg(m.c_str()); }
+
+bool check_ptrs(std::initializer_list<std::pair<const void*, const char*>> ptrs) {
+    for (const auto& p : ptrs) {
+        if (!p.first) {
+            char buf[256];
+            std::snprintf(buf, sizeof buf, "null pointer: %s", p.second);
+            set_msg(buf);
+            return false;
+        }
+    }
+    return true;
+}
+
+bool check_n(int32_t n) {
+    if (n <= 0) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "bad batch size: %d", n);
+        set_msg(buf);
+        return false;
+    }
+    return true;
+}
+
+#define QK_P(x) std::pair<const void*, const char*>(static_cast<const void*>(x), #x)
+
+const QKPluginAPI k_api = {QK_ABI_MAJOR, QK_ABI_MINOR,
+                           "quantkernel.mi355x.gfx950.mc_batch.v1"};
+
+int32_t run(const qk::HostBatch& hb, int32_t n, double* out, double* se) {
+    std::string err;
+    const int32_t rc = qk::run_mc_batch(hb, n, out, se, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
+// GBM families sharing (spot, strike, t, vol, r, q, ot, paths[, steps], seed).
+int32_t gbm_batch(int32_t method, const double* spot, const double* strike, const double* t,
+                  const double* vol, const double* r, const double* q, const int32_t* ot,
+                  const int32_t* paths, const int32_t* steps, const uint64_t* seed, int32_t n,
+                  double* out, double* se) {
+    qk::HostBatch hb;
+    hb.uniform_method = method;
+    hb.spot = spot;
+    hb.strike = strike;
+    hb.t = t;
+    hb.vol = vol;
+    hb.r = r;
+    hb.q = q;
+    hb.ot = ot;
+    hb.paths = paths;
+    hb.steps = steps;
+    hb.seed = seed;
+    return run(hb, n, out, se);
+}
+
+int32_t heston_batch(const double* spot, const double* strike, const double* t, const double* r,
+                     const double* q, const double* v0, const double* kappa, const double* theta,
+                     const double* sigma, const double* rho, const int32_t* ot,
+                     const int32_t* paths, const int32_t* steps, const uint64_t* seed, int32_t n,
+                     double* out, double* se) {
+    qk::HostBatch hb;
+    hb.uniform_method = qk::M_HESTON;
+    hb.spot = spot;
+    hb.strike = strike;
+    hb.t = t;
+    hb.r = r;
+    hb.q = q;
+    hb.v0 = v0;
+    hb.kappa = kappa;
+    hb.theta = theta;
+    hb.sigma = sigma;
+    hb.rho = rho;
+    hb.ot = ot;
+    hb.paths = paths;
+    hb.steps = steps;
+    hb.seed = seed;
+    return run(hb, n, out, se);
+}
+
+double scalar_of(int32_t rc, double v) { return rc == QK_OK ? v : qk::qnan_d(); }
+
+}  // namespace
+
+extern "C" {
+
+// ---- ABI negotiation and errors (reference qk_api.cpp:94-117) -------------
+void qk_abi_version(int32_t* major, int32_t* minor) {
+    if (major) *major = QK_ABI_MAJOR;
+    if (minor) *minor = QK_ABI_MINOR;
+}
+
+int32_t qk_plugin_get_api(int32_t host_abi_major, int32_t host_abi_minor,
+                          const QKPluginAPI** out_api) {
+    if (!out_api) return QK_ERR_NULL_PTR;
+    if (host_abi_major != QK_ABI_MAJOR || host_abi_minor > QK_ABI_MINOR) {
+        *out_api = nullptr;
+        return QK_ERR_ABI_MISMATCH;
+    }
+    *out_api = &k_api;
+    return QK_OK;
+}
+
+const char* qk_get_last_error(void) { return tl_msg; }
+
+void qk_clear_last_error(void) { tl_msg[0] = '\0'; }
+
+// ---- standard MC (reference qk_api.cpp:837-867) ---------------------------
+int32_t qk_mcm_standard_monte_carlo_price_batch_se(const double* spot, const double* strike,
+                                                   const double* t, const double* vol,
+                                                   const double* r, const double* q,
+                                                   const int32_t* option_type,
+                                                   const int32_t* paths, const uint64_t* seed,
+                                                   int32_t n, double* out_prices,
+                                                   double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_batch(qk::M_STANDARD, spot, strike, t, vol, r, q, option_type, paths, nullptr, seed,
+                     n, out_prices, out_stderr);
+}
+
+int32_t qk_mcm_standard_monte_carlo_price_batch(const double* spot, const double* strike,
+                                                const double* t, const double* vol,
+                                                const double* r, const double* q,
+                                                const int32_t* option_type, const int32_t* paths,
+                                                const uint64_t* seed, int32_t n,
+                                                double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_batch(qk::M_STANDARD, spot, strike, t, vol, r, q, option_type, paths, nullptr, seed,
+                     n, out_prices, nullptr);
+}
+
+double qk_mcm_standard_monte_carlo_price(double spot, double strike, double t, double vol,
+                                         double r, double q, int32_t option_type, int32_t paths,
+                                         uint64_t seed) {
+    double out = 0.0;
+    const int32_t rc = gbm_batch(qk::M_STANDARD, &spot, &strike, &t, &vol, &r, &q, &option_type,
+                                 &paths, nullptr, &seed, 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- antithetic variates (reference qk_api.cpp:1091-1107) -----------------
+int32_t qk_mcm_antithetic_variates_price_batch(const double* spot, const double* strike,
+                                               const double* t, const double* vol,
+                                               const double* r, const double* q,
+                                               const int32_t* option_type, const int32_t* paths,
+                                               const uint64_t* seed, int32_t n,
+                                               double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_batch(qk::M_ANTITHETIC, spot, strike, t, vol, r, q, option_type, paths, nullptr,
+                     seed, n, out_prices, nullptr);
+}
+
+double qk_mcm_antithetic_variates_price(double spot, double strike, double t, double vol,
+                                        double r, double q, int32_t option_type, int32_t paths,
+                                        uint64_t seed) {
+    double out = 0.0;
+    const int32_t rc = gbm_batch(qk::M_ANTITHETIC, &spot, &strike, &t, &vol, &r, &q, &option_type,
+                                 &paths, nullptr, &seed, 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- stepwise GBM families (reference qk_api.cpp:950-984, 1210-1227) ------
+#define QK_STEPWISE_ENTRY(NAME, METHOD)                                                          \
+    int32_t NAME##_batch(const double* spot, const double* strike, const double* t,              \
+                         const double* vol, const double* r, const double* q,                    \
+                         const int32_t* option_type, const int32_t* paths,                       \
+                         const int32_t* steps, const uint64_t* seed, int32_t n,                  \
+                         double* out_prices) {                                                   \
+        if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),         \
+                         QK_P(option_type), QK_P(paths), QK_P(steps), QK_P(seed),                \
+                         QK_P(out_prices)}))                                                     \
+            return QK_ERR_NULL_PTR;                                                              \
+        if (!check_n(n)) return QK_ERR_BAD_SIZE;                                                 \
+        return gbm_batch(METHOD, spot, strike, t, vol, r, q, option_type, paths, steps, seed, n, \
+                         out_prices, nullptr);                                                   \
+    }                                                                                            \
+    int32_t NAME##_batch_se(const double* spot, const double* strike, const double* t,           \
+                            const double* vol, const double* r, const double* q,                 \
+                            const int32_t* option_type, const int32_t* paths,                    \
+                            const int32_t* steps, const uint64_t* seed, int32_t n,               \
+                            double* out_prices, double* out_stderr) {                            \
+        if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),         \
+                         QK_P(option_type), QK_P(paths), QK_P(steps), QK_P(seed),                \
+                         QK_P(out_prices), QK_P(out_stderr)}))                                   \
+            return QK_ERR_NULL_PTR;                                                              \
+        if (!check_n(n)) return QK_ERR_BAD_SIZE;                                                 \
+        return gbm_batch(METHOD, spot, strike, t, vol, r, q, option_type, paths, steps, seed, n, \
+                         out_prices, out_stderr);                                                \
+    }                                                                                            \
+    double NAME(double spot, double strike, double t, double vol, double r, double q,            \
+                int32_t option_type, int32_t paths, int32_t steps, uint64_t seed) {              \
+        double out = 0.0;                                                                        \
+        const int32_t rc = gbm_batch(METHOD, &spot, &strike, &t, &vol, &r, &q, &option_type,     \
+                                     &paths, &steps, &seed, 1, &out, nullptr);                   \
+        return scalar_of(rc, out);                                                               \
+    }
+
+QK_STEPWISE_ENTRY(qk_mcm_euler_maruyama_price, qk::M_EULER)
+QK_STEPWISE_ENTRY(qk_mcm_milstein_price, qk::M_MILSTEIN)
+QK_STEPWISE_ENTRY(qk_mcm_local_vol_monte_carlo_price, qk::M_LOCAL_VOL)
+
+// ---- Heston (reference qk_api.cpp:1127-1161) ------------------------------
+int32_t qk_mcm_heston_monte_carlo_price_batch(const double* spot, const double* strike,
+                                              const double* t, const double* r, const double* q,
+                                              const double* v0, const double* kappa,
+                                              const double* theta, const double* sigma,
+                                              const double* rho, const int32_t* option_type,
+                                              const int32_t* paths, const int32_t* steps,
+                                              const uint64_t* seed, int32_t n,
+                                              double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(r), QK_P(q), QK_P(v0), QK_P(kappa),
+                     QK_P(theta), QK_P(sigma), QK_P(rho), QK_P(option_type), QK_P(paths),
+                     QK_P(steps), QK_P(seed), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return heston_batch(spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type, paths,
+                        steps, seed, n, out_prices, nullptr);
+}
+
+int32_t qk_mcm_heston_monte_carlo_price_batch_se(const double* spot, const double* strike,
+                                                 const double* t, const double* r,
+                                                 const double* q, const double* v0,
+                                                 const double* kappa, const double* theta,
+                                                 const double* sigma, const double* rho,
+                                                 const int32_t* option_type, const int32_t* paths,
+                                                 const int32_t* steps, const uint64_t* seed,
+                                                 int32_t n, double* out_prices,
+                                                 double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(r), QK_P(q), QK_P(v0), QK_P(kappa),
+                     QK_P(theta), QK_P(sigma), QK_P(rho), QK_P(option_type), QK_P(paths),
+                     QK_P(steps), QK_P(seed), QK_P(out_prices), QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return heston_batch(spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type, paths,
+                        steps, seed, n, out_prices, out_stderr);
+}
+
+double qk_mcm_heston_monte_carlo_price(double spot, double strike, double t, double r, double q,
+                                       double v0, double kappa, double theta, double sigma,
+                                       double rho, int32_t option_type, int32_t paths,
+                                       int32_t steps, uint64_t seed) {
+    double out = 0.0;
+    const int32_t rc = heston_batch(&spot, &strike, &t, &r, &q, &v0, &kappa, &theta, &sigma, &rho,
+                                    &option_type, &paths, &steps, &seed, 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- mixed-model batches (ABI 2.13) ---------------------------------------
+int32_t qk_mcm_mixed_price_batch(const int32_t* model, const double* spot, const double* strike,
+                                 const double* t, const double* vol, const double* r,
+                                 const double* q, const double* v0, const double* kappa,
+                                 const double* theta, const double* sigma, const double* rho,
+                                 const int32_t* option_type, const int32_t* paths,
+                                 const int32_t* steps, const uint64_t* seed, int32_t n,
+                                 double* out_prices, double* out_stderr) {
+    if (!check_ptrs({QK_P(model), QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(v0), QK_P(kappa), QK_P(theta), QK_P(sigma), QK_P(rho),
+                     QK_P(option_type), QK_P(paths), QK_P(steps), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    qk::HostBatch hb;
+    hb.model_ids = model;
+    hb.spot = spot;
+    hb.strike = strike;
+    hb.t = t;
+    hb.vol = vol;
+    hb.r = r;
+    hb.q = q;
+    hb.v0 = v0;
+    hb.kappa = kappa;
+    hb.theta = theta;
+    hb.sigma = sigma;
+    hb.rho = rho;
+    hb.ot = option_type;
+    hb.paths = paths;
+    hb.steps = steps;
+    hb.seed = seed;
+    return run(hb, n, out_prices, out_stderr);
+}
+
+int32_t qk_mcm_mixed_price_batch_device(const int32_t* model, const double* spot,
+                                        const double* strike, const double* t, const double* vol,
+                                        const double* r, const double* q, const double* v0,
+                                        const double* kappa, const double* theta,
+                                        const double* sigma, const double* rho,
+                                        const int32_t* option_type, const int32_t* paths,
+                                        const int32_t* steps, const uint64_t* seed, int32_t n,
+                                        double* out_prices, double* out_stderr) {
+    if (!check_ptrs({QK_P(model), QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(v0), QK_P(kappa), QK_P(theta), QK_P(sigma), QK_P(rho),
+                     QK_P(option_type), QK_P(paths), QK_P(steps), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    qk::BatchIn bi{};
+    bi.method = model;
+    bi.method_is_model = 1;
+    bi.spot = spot;
+    bi.strike = strike;
+    bi.t = t;
+    bi.vol = vol;
+    bi.r = r;
+    bi.q = q;
+    bi.v0 = v0;
+    bi.kappa = kappa;
+    bi.theta = theta;
+    bi.sigma = sigma;
+    bi.rho = rho;
+    bi.ot = option_type;
+    bi.paths = paths;
+    bi.steps = steps;
+    bi.seed = seed;
+    std::string err;
+    const int32_t rc = qk::run_mc_batch_device(bi, n, out_prices, out_stderr, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
+// ---- RNG mode, timing, device info (ABI 2.13) -----------------------------
+int32_t qk_mc_set_rng_mode(int32_t mode) {
+    if (mode != QK_RNG_PHILOX && mode != QK_RNG_MT19937_FP64 && mode != QK_RNG_MT19937_FP32) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "unknown rng mode: %d", mode);
+        set_msg(buf);
+        return QK_ERR_INVALID_INPUT;
+    }
+    qk::set_rng_mode(mode);
+    return QK_OK;
+}
+
+int32_t qk_mc_get_rng_mode(void) { return qk::get_rng_mode(); }
+
+int32_t qk_mc_last_timing(double* ms_total, double* ms_simulate) {
+    if (!qk::last_timing(ms_total, ms_simulate)) {
+        set_msg("no MC batch has completed on the device yet");
+        return QK_ERR_RUNTIME_INIT;
+    }
+    return QK_OK;
+}
+
+int32_t qk_device_info(char* name_buf, int32_t buf_len, int32_t* compute_units) {
+    std::string name, err;
+    int32_t cus = 0;
+    if (!qk::device_info(&name, &cus, &err)) {
+        set_msg(err);
+        return QK_ERR_RUNTIME_INIT;
+    }
+    if (name_buf && buf_len > 0) {
+        std::strncpy(name_buf, name.c_str(), size_t(buf_len) - 1);
+        name_buf[buf_len - 1] = '\0';
+    }
+    if (compute_units) *compute_units = cus;
+    return QK_OK;
+}
+
+int32_t qk_mc_validation_stream(uint64_t seed, int64_t count, double* out) {
+    if (!check_ptrs({QK_P(out)})) return QK_ERR_NULL_PTR;
+    if (count < 0) {
+        set_msg("bad stream length");
+        return QK_ERR_BAD_SIZE;
+    }
+    qk::host_normal_stream(seed, count, out);
+    return QK_OK;
+}
+
+}  // extern "C"
+
+// ---- Longstaff-Schwartz (reference qk_api.cpp:986-1002, 875-880) ----------
+extern "C" {
+
+int32_t qk_mcm_longstaff_schwartz_price_batch_ex(const double* spot, const double* strike,
+                                                 const double* t, const double* vol,
+                                                 const double* r, const double* q,
+                                                 const int32_t* option_type, const int32_t* paths,
+                                                 const int32_t* steps, const uint64_t* seed,
+                                                 int32_t basis, int32_t n, double* out_prices,
+                                                 double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(steps), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    if (basis != QK_LSMC_BASIS_REFERENCE && basis != QK_LSMC_BASIS_CUBIC) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "unknown LSMC basis: %d", basis);
+        set_msg(buf);
+        return QK_ERR_INVALID_INPUT;
+    }
+    std::string err;
+    const int32_t rc = qk::run_lsmc_batch(spot, strike, t, vol, r, q, option_type, paths, steps,
+                                          seed, basis, n, out_prices, out_stderr, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
+int32_t qk_mcm_longstaff_schwartz_price_batch(const double* spot, const double* strike,
+                                              const double* t, const double* vol,
+                                              const double* r, const double* q,
+                                              const int32_t* option_type, const int32_t* paths,
+                                              const int32_t* steps, const uint64_t* seed,
+                                              int32_t n, double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(steps), QK_P(seed), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    std::string err;
+    const int32_t rc =
+        qk::run_lsmc_batch(spot, strike, t, vol, r, q, option_type, paths, steps, seed,
+                           QK_LSMC_BASIS_REFERENCE, n, out_prices, nullptr, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
+double qk_mcm_longstaff_schwartz_price(double spot, double strike, double t, double vol,
+                                       double r, double q, int32_t option_type, int32_t paths,
+                                       int32_t steps, uint64_t seed) {
+    double out = 0.0;
+    std::string err;
+    const int32_t rc = qk::run_lsmc_batch(&spot, &strike, &t, &vol, &r, &q, &option_type, &paths,
+                                          &steps, &seed, QK_LSMC_BASIS_REFERENCE, 1, &out,
+                                          nullptr, &err);
+    if (rc != QK_OK) set_msg(err);
+    return scalar_of(rc, out);
+}
+
+}  // extern "C"

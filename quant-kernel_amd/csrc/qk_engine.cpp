@@ -1,0 +1,627 @@
+// qk_engine.cpp - host engine of the MI355X Monte Carlo batch-pricing path.
+//
+// One lazily created device context per process (std::call_once): the HIP
+// device, one default-priority stream, timing events and grow-only device
+// and pinned host buffers.  Device work is serialised by a mutex; the C ABI
+// may be entered from several host threads.
+#include "qk_engine.hpp"
+#include "qk_lsmc.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <random>
+#include <thread>
+#include <vector>
+
+namespace qk {
+
+namespace {
+
+std::atomic<int32_t> g_rng_mode{QK_RNG_PHILOX};
+
+#define QK_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            if (err) *err = std::string("HIP error: ") + hipGetErrorString(e_) + " (" + \
+                            #call + ")";                                               \
+            return QK_ERR_RUNTIME_INIT;                                                \
+        }                                                                              \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 4096);
+        want = (want + 4095) & ~size_t(4095);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+};
+
+// Grow-only pinned host buffer.
+struct PinBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipHostFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 4096);
+        want = (want + 4095) & ~size_t(4095);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+};
+
+int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return std::atoi(v);
+}
+
+struct Device {
+    int dev = 0;
+    int cus = 0;
+    std::string name;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    std::mutex mu;
+
+    DevBuf in, recs, ntiles, tile_start, counters, partials, out, stream_buf, stream_off;
+    DevBuf lsmc_path, lsmc_cf, lsmc_recs;
+    PinBuf pin_in, pin_out;
+
+    double ms_total = -1.0, ms_sim = -1.0;
+
+    SimShape shape_step{8, 0};
+    SimShape shape_term{8, 0};
+};
+
+Device* g_dev = nullptr;
+std::string g_init_err;
+std::once_flag g_once;
+
+void init_device() {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        g_init_err = std::string("no HIP device available (") +
+                     (e != hipSuccess ? hipGetErrorString(e) : "0 devices") + ")";
+        return;
+    }
+    const int want = env_int("QK_DEVICE", 0);
+    if (want < 0 || want >= count) {
+        g_init_err = "QK_DEVICE out of range";
+        return;
+    }
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, want);
+    if (e != hipSuccess) {
+        g_init_err = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+        return;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_init_err = std::string("device ") + prop.gcnArchName +
+                     " is not gfx950 (this library is built for MI355X only)";
+        return;
+    }
+    e = hipSetDevice(want);
+    if (e != hipSuccess) {
+        g_init_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return;
+    }
+    Device* d = new Device();
+    d->dev = want;
+    d->cus = prop.multiProcessorCount;
+    d->name = prop.name;
+    e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&d->ev[i]);
+    if (e != hipSuccess) {
+        g_init_err = std::string("stream/event creation: ") + hipGetErrorString(e);
+        delete d;
+        return;
+    }
+    const int per_cu = std::max(1, env_int("QK_GRID_PER_CU", 8));
+    d->shape_step.lane_units = std::max(1, env_int("QK_LANE_PATHS", 8));
+    d->shape_step.grid = d->cus * per_cu;
+    d->shape_term.lane_units = std::max(1, env_int("QK_LANE_GROUPS", 8));
+    d->shape_term.grid = d->cus * per_cu;
+    g_dev = d;
+}
+
+Device* get_device(std::string* err) {
+    std::call_once(g_once, init_device);
+    if (!g_dev && err) *err = g_init_err;
+    return g_dev;
+}
+
+int32_t method_of(const HostBatch& hb, int32_t i) {
+    if (!hb.model_ids) return hb.uniform_method;
+    switch (hb.model_ids[i]) {
+        case QK_MODEL_GBM_EULER: return M_EULER;
+        case QK_MODEL_GBM_MILSTEIN: return M_MILSTEIN;
+        case QK_MODEL_HESTON_EULER: return M_HESTON;
+        case QK_MODEL_LOCAL_VOL: return M_LOCAL_VOL;
+        default: return -1;
+    }
+}
+
+bool is_terminal_method(int32_t m) { return m == M_STANDARD || m == M_ANTITHETIC; }
+
+size_t align64(size_t x) { return (x + 63) & ~size_t(63); }
+
+// Launch prepare -> scan -> simulate -> finalize on d's stream.  The caller
+// holds d->mu.  `family` selects the simulation kernel.
+int32_t launch_pipeline(Device* d, const BatchIn& in, const int32_t* n_dev,
+                        const int64_t* stream_off, int32_t family, int32_t mode,
+                        const void* stream, double* out_price, double* out_se,
+                        std::string* err) {
+    OptionRec* recs = static_cast<OptionRec*>(d->recs.p);
+    int32_t* ntiles = static_cast<int32_t*>(d->ntiles.p);
+    int32_t* tstart = static_cast<int32_t*>(d->tile_start.p);
+    int32_t* ctr = static_cast<int32_t*>(d->counters.p);
+    Partial* parts = static_cast<Partial*>(d->partials.p);
+    const SimShape& shp = family == FAM_TERMINAL ? d->shape_term : d->shape_step;
+    QK_HIP(hipEventRecord(d->ev[0], d->stream));
+    QK_HIP(launch_prepare(d->stream, in, n_dev, stream_off, tile_units_terminal(d->shape_term),
+                          tile_units_stepwise(d->shape_step), recs, ntiles));
+    QK_HIP(launch_scan(d->stream, n_dev, ntiles, recs, tstart, ctr));
+    QK_HIP(hipEventRecord(d->ev[1], d->stream));
+    QK_HIP(launch_simulate(d->stream, family, mode, shp, recs, tstart, n_dev, ctr, parts, stream));
+    QK_HIP(hipEventRecord(d->ev[2], d->stream));
+    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, out_price, out_se));
+    QK_HIP(hipEventRecord(d->ev[3], d->stream));
+    return QK_OK;
+}
+
+void record_timing(Device* d) {
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, d->ev[0], d->ev[3]) == hipSuccess &&
+        hipEventElapsedTime(&b, d->ev[1], d->ev[2]) == hipSuccess) {
+        d->ms_total = a;
+        d->ms_sim = b;
+    }
+}
+
+// Tile-count bound from the raw path counts (valid elements can only need
+// fewer tiles than this).
+int64_t tile_bound(const HostBatch& hb, int32_t n, const Device* d) {
+    const int64_t tu_s = tile_units_stepwise(d->shape_step);
+    const int64_t tu_t = tile_units_terminal(d->shape_term);
+    int64_t total = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const int64_t p = std::max<int32_t>(hb.paths[i], 0);
+        const int32_t m = method_of(hb, i);
+        if (m < 0) continue;
+        if (is_terminal_method(m)) total += (p / 2 + tu_t - 1) / tu_t + 1;
+        else total += (p + tu_s - 1) / tu_s;
+    }
+    return total;
+}
+
+}  // namespace
+
+void set_rng_mode(int32_t mode) { g_rng_mode.store(mode); }
+int32_t get_rng_mode() { return g_rng_mode.load(); }
+
+void host_normal_stream(uint64_t seed, int64_t count, double* out) {
+    std::mt19937_64 eng(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (int64_t i = 0; i < count; ++i) out[i] = nd(eng);
+}
+
+bool last_timing(double* ms_total, double* ms_simulate) {
+    if (!g_dev || g_dev->ms_total < 0.0) return false;
+    if (ms_total) *ms_total = g_dev->ms_total;
+    if (ms_simulate) *ms_simulate = g_dev->ms_sim;
+    return true;
+}
+
+bool device_info(std::string* name, int32_t* cus, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return false;
+    if (name) *name = d->name;
+    if (cus) *cus = d->cus;
+    return true;
+}
+
+// Builds the validation-mode stream: per option the host mt19937_64 normal
+// sequence in the reference's draw order (paths outer, draws inner;
+// mc_engine.h:84-136), stored draw-major for the kernels.
+template <typename T>
+static void fill_streams(const HostBatch& hb, int32_t n, const std::vector<OptionRec>& recs,
+                         const std::vector<int64_t>& off, T* buf) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::atomic<int32_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int32_t i = next.fetch_add(1);
+            if (i >= n) break;
+            const OptionRec& rc = recs[i];
+            if (rc.family == FAM_NONE) continue;
+            std::mt19937_64 eng(hb.seed[i]);
+            std::normal_distribution<double> nd(0.0, 1.0);
+            T* base = buf + off[i];
+            if (rc.family == FAM_TERMINAL) {
+                const int64_t cnt = stream_normals(rc);
+                for (int64_t k = 0; k < cnt; ++k) base[k] = T(nd(eng));
+            } else {
+                const int64_t P = rc.paths;
+                const int64_t D = int64_t(rc.steps) * (rc.method == M_HESTON ? 2 : 1);
+                for (int64_t p = 0; p < P; ++p)
+                    for (int64_t dd = 0; dd < D; ++dd) base[dd * P + p] = T(nd(eng));
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned k = 1; k < hw; ++k) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+}
+
+int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* out_se,
+                     std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+
+    const int32_t mode = get_rng_mode();
+    // family of the launch: a batch is all-terminal or all-stepwise
+    int32_t family = FAM_STEPWISE;
+    if (!hb.model_ids && is_terminal_method(hb.uniform_method)) family = FAM_TERMINAL;
+
+    const int64_t tiles = tile_bound(hb, n, d);
+    if (tiles > 0x7fffffffLL) {
+        if (err) *err = "batch too large: more than 2^31-1 path tiles";
+        return QK_ERR_INVALID_INPUT;
+    }
+
+    // ---- pack inputs (SoA, stride ns) into pinned memory, one H2D copy ----
+    const size_t ns = align64(size_t(n));
+    const double* dsrc[11] = {hb.spot, hb.strike, hb.t, hb.vol, hb.r, hb.q,
+                              hb.v0, hb.kappa, hb.theta, hb.sigma, hb.rho};
+    const int32_t* isrc[4] = {hb.model_ids, hb.ot, hb.paths, hb.steps};
+    const size_t in_bytes = ns * (11 * 8 + 8 + 4 * 4) + 64;
+    QK_HIP(d->pin_in.ensure(in_bytes));
+    QK_HIP(d->in.ensure(in_bytes));
+    char* hp = static_cast<char*>(d->pin_in.p);
+    char* dp = static_cast<char*>(d->in.p);
+    BatchIn bi{};
+    const double** dfield[11] = {&bi.spot, &bi.strike, &bi.t, &bi.vol, &bi.r, &bi.q,
+                                 &bi.v0, &bi.kappa, &bi.theta, &bi.sigma, &bi.rho};
+    size_t off = 0;
+    for (int k = 0; k < 11; ++k) {
+        if (dsrc[k]) {
+            std::memcpy(hp + off, dsrc[k], size_t(n) * 8);
+            *dfield[k] = reinterpret_cast<const double*>(dp + off);
+        } else {
+            *dfield[k] = nullptr;
+        }
+        off += ns * 8;
+    }
+    std::memcpy(hp + off, hb.seed, size_t(n) * 8);
+    bi.seed = reinterpret_cast<const uint64_t*>(dp + off);
+    off += ns * 8;
+    const int32_t** ifield[4] = {&bi.method, &bi.ot, &bi.paths, &bi.steps};
+    for (int k = 0; k < 4; ++k) {
+        if (isrc[k]) {
+            std::memcpy(hp + off, isrc[k], size_t(n) * 4);
+            *ifield[k] = reinterpret_cast<const int32_t*>(dp + off);
+        } else {
+            *ifield[k] = nullptr;
+        }
+        off += ns * 4;
+    }
+    std::memcpy(hp + off, &n, 4);
+    const int32_t* n_dev = reinterpret_cast<const int32_t*>(dp + off);
+    off += 64;
+    bi.uniform_method = hb.uniform_method;
+    bi.method_is_model = hb.model_ids ? 1 : 0;
+
+    // ---- device work buffers ----
+    QK_HIP(d->recs.ensure(ns * sizeof(OptionRec)));
+    QK_HIP(d->ntiles.ensure(ns * 4));
+    QK_HIP(d->tile_start.ensure((ns + 1) * 4));
+    QK_HIP(d->counters.ensure(64));
+    QK_HIP(d->partials.ensure(size_t(std::max<int64_t>(tiles, 1)) * sizeof(Partial)));
+    QK_HIP(d->out.ensure(ns * 16 + 64));
+    QK_HIP(d->pin_out.ensure(ns * 16 + 64));
+    double* dprice = static_cast<double*>(d->out.p);
+    double* dse = dprice + ns;
+
+    // ---- validation modes: host normal stream ----
+    const int64_t* d_off = nullptr;
+    const void* d_stream = nullptr;
+    int32_t sim_mode = SIM_PHILOX_F32;
+    if (mode == QK_RNG_MT19937_FP64 || mode == QK_RNG_MT19937_FP32) {
+        sim_mode = mode == QK_RNG_MT19937_FP64 ? SIM_STREAM_F64 : SIM_STREAM_F32;
+        std::vector<OptionRec> hrec(static_cast<size_t>(n));
+        std::vector<int64_t> hoff(static_cast<size_t>(n) + 1, 0);
+        for (int32_t i = 0; i < n; ++i) {
+            RawOption o;
+            o.method = method_of(hb, i);
+            o.spot = hb.spot[i];
+            o.strike = hb.strike[i];
+            o.t = hb.t[i];
+            o.vol = hb.vol ? hb.vol[i] : 0.0;
+            o.r = hb.r[i];
+            o.q = hb.q[i];
+            o.v0 = hb.v0 ? hb.v0[i] : 0.0;
+            o.kappa = hb.kappa ? hb.kappa[i] : 0.0;
+            o.theta = hb.theta ? hb.theta[i] : 0.0;
+            o.sigma = hb.sigma ? hb.sigma[i] : 0.0;
+            o.rho = hb.rho ? hb.rho[i] : 0.0;
+            o.ot = hb.ot[i];
+            o.paths = hb.paths[i];
+            o.steps = hb.steps ? hb.steps[i] : 1;
+            o.seed = hb.seed[i];
+            if (o.method < 0) {
+                hrec[i].family = FAM_NONE;
+            } else {
+                prepare_option(o, 1, hrec[i]);
+            }
+            hoff[i + 1] = hoff[i] + stream_normals(hrec[i]);
+        }
+        const int64_t total = hoff[n];
+        const size_t esz = sim_mode == SIM_STREAM_F64 ? 8 : 4;
+        const int64_t limit = int64_t(env_int("QK_VALIDATION_STREAM_GB", 32)) << 30;
+        if (total * int64_t(esz) > limit) {
+            if (err)
+                *err = "validation stream too large (" + std::to_string(total) +
+                       " normals); reduce paths or raise QK_VALIDATION_STREAM_GB";
+            return QK_ERR_INVALID_INPUT;
+        }
+        QK_HIP(d->stream_buf.ensure(size_t(std::max<int64_t>(total, 1)) * esz));
+        QK_HIP(d->stream_off.ensure(ns * 8));
+        std::vector<char> host(size_t(std::max<int64_t>(total, 1)) * esz);
+        if (sim_mode == SIM_STREAM_F64)
+            fill_streams<double>(hb, n, hrec, hoff, reinterpret_cast<double*>(host.data()));
+        else
+            fill_streams<float>(hb, n, hrec, hoff, reinterpret_cast<float*>(host.data()));
+        QK_HIP(hipMemcpyAsync(d->stream_buf.p, host.data(), size_t(total) * esz,
+                              hipMemcpyHostToDevice, d->stream));
+        QK_HIP(hipMemcpyAsync(d->stream_off.p, hoff.data(), size_t(n) * 8, hipMemcpyHostToDevice,
+                              d->stream));
+        QK_HIP(hipStreamSynchronize(d->stream));  // host vectors die at scope end
+        d_off = static_cast<const int64_t*>(d->stream_off.p);
+        d_stream = d->stream_buf.p;
+    }
+
+    QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, off, hipMemcpyHostToDevice, d->stream));
+    int32_t rc = launch_pipeline(d, bi, n_dev, d_off, family, sim_mode, d_stream, dprice, dse, err);
+    if (rc != QK_OK) return rc;
+    // outputs + the scan's overflow flag
+    QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, ns * 16, hipMemcpyDeviceToHost, d->stream));
+    QK_HIP(hipMemcpyAsync(static_cast<char*>(d->pin_out.p) + ns * 16, d->counters.p, 16,
+                          hipMemcpyDeviceToHost, d->stream));
+    QK_HIP(hipStreamSynchronize(d->stream));
+    record_timing(d);
+    const int32_t* hc = reinterpret_cast<const int32_t*>(static_cast<char*>(d->pin_out.p) + ns * 16);
+    if (hc[2]) {
+        if (err) *err = "batch too large: more than 2^31-1 path tiles";
+        return QK_ERR_INVALID_INPUT;
+    }
+    const double* hpr = static_cast<const double*>(d->pin_out.p);
+    std::memcpy(out_price, hpr, size_t(n) * 8);
+    if (out_se) std::memcpy(out_se, hpr + ns, size_t(n) * 8);
+    return QK_OK;
+}
+
+int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
+                            double* out_se_dev, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    if (get_rng_mode() != QK_RNG_PHILOX) {
+        if (err) *err = "device-resident batches support QK_RNG_PHILOX only";
+        return QK_ERR_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+    const size_t ns = align64(size_t(n));
+    QK_HIP(d->recs.ensure(ns * sizeof(OptionRec)));
+    QK_HIP(d->ntiles.ensure(ns * 4));
+    QK_HIP(d->tile_start.ensure((ns + 1) * 4));
+    QK_HIP(d->counters.ensure(64));
+    QK_HIP(d->pin_in.ensure(64));
+    QK_HIP(d->in.ensure(64));
+    QK_HIP(d->pin_out.ensure(64));
+    std::memcpy(d->pin_in.p, &n, 4);
+    QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, 4, hipMemcpyHostToDevice, d->stream));
+    const int32_t* n_dev = static_cast<const int32_t*>(d->in.p);
+    // tile count is only known on the device: prepare + scan, read it back,
+    // size the partials, then simulate.
+    OptionRec* recs = static_cast<OptionRec*>(d->recs.p);
+    int32_t* ntiles = static_cast<int32_t*>(d->ntiles.p);
+    int32_t* tstart = static_cast<int32_t*>(d->tile_start.p);
+    int32_t* ctr = static_cast<int32_t*>(d->counters.p);
+    QK_HIP(hipEventRecord(d->ev[0], d->stream));
+    QK_HIP(launch_prepare(d->stream, in, n_dev, nullptr, tile_units_terminal(d->shape_term),
+                          tile_units_stepwise(d->shape_step), recs, ntiles));
+    QK_HIP(launch_scan(d->stream, n_dev, ntiles, recs, tstart, ctr));
+    QK_HIP(hipMemcpyAsync(d->pin_out.p, ctr, 16, hipMemcpyDeviceToHost, d->stream));
+    QK_HIP(hipStreamSynchronize(d->stream));
+    const int32_t* hc = static_cast<const int32_t*>(d->pin_out.p);
+    if (hc[2]) {
+        if (err) *err = "batch too large: more than 2^31-1 path tiles";
+        return QK_ERR_INVALID_INPUT;
+    }
+    QK_HIP(d->partials.ensure(size_t(std::max(hc[1], 1)) * sizeof(Partial)));
+    const int32_t family =
+        (!in.method && is_terminal_method(in.uniform_method)) ? FAM_TERMINAL : FAM_STEPWISE;
+    const SimShape& shp = family == FAM_TERMINAL ? d->shape_term : d->shape_step;
+    Partial* parts = static_cast<Partial*>(d->partials.p);
+    QK_HIP(hipEventRecord(d->ev[1], d->stream));
+    QK_HIP(launch_simulate(d->stream, family, SIM_PHILOX_F32, shp, recs, tstart, n_dev, ctr, parts,
+                           nullptr));
+    QK_HIP(hipEventRecord(d->ev[2], d->stream));
+    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, out_price_dev, out_se_dev));
+    QK_HIP(hipEventRecord(d->ev[3], d->stream));
+    QK_HIP(hipStreamSynchronize(d->stream));
+    record_timing(d);
+    return QK_OK;
+}
+
+// Longstaff-Schwartz batch: host validation (longstaff_schwartz.cpp:50-55),
+// then chunks of options sized to a memory budget, one workgroup per option.
+int32_t run_lsmc_batch(const double* spot, const double* strike, const double* t,
+                       const double* vol, const double* r, const double* q, const int32_t* ot,
+                       const int32_t* paths, const int32_t* steps, const uint64_t* seed,
+                       int32_t basis, int32_t n, double* out_price, double* out_se,
+                       std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+    const int32_t mode = get_rng_mode();
+    const int32_t sim_mode = mode == QK_RNG_MT19937_FP64   ? SIM_STREAM_F64
+                             : mode == QK_RNG_MT19937_FP32 ? SIM_STREAM_F32
+                                                           : SIM_PHILOX_F32;
+    const size_t esz = sim_mode == SIM_STREAM_F64 ? 8 : 4;
+
+    std::vector<LsmcRec> all;
+    all.reserve(size_t(n));
+    for (int32_t i = 0; i < n; ++i) {
+        out_price[i] = qnan_d();
+        if (out_se) out_se[i] = qnan_d();
+        if (!common_ok(spot[i], strike[i], t[i], vol[i], r[i], q[i], ot[i]) || paths[i] <= 1 ||
+            steps[i] <= 0)
+            continue;
+        if (t[i] <= kEps) {
+            out_price[i] = intrinsic(spot[i], strike[i], ot[i]);
+            if (out_se) out_se[i] = 0.0;
+            continue;
+        }
+        LsmcRec rc{};
+        rc.out_index = i;
+        rc.ot = ot[i];
+        rc.paths = paths[i];
+        rc.steps = steps[i];
+        rc.seed_lo = uint32_t(seed[i]);
+        rc.seed_hi = uint32_t(seed[i] >> 32);
+        const double dt = t[i] / double(steps[i]);
+        rc.spot = spot[i];
+        rc.strike = strike[i];
+        rc.disc = std::exp(-r[i] * dt);
+        rc.drift = (r[i] - q[i] - 0.5 * vol[i] * vol[i]) * dt;
+        rc.vs = vol[i] * std::sqrt(dt);
+        all.push_back(rc);
+    }
+    if (all.empty()) return QK_OK;
+
+    const size_t ns = align64(size_t(n));
+    QK_HIP(d->out.ensure(ns * 16 + 64));
+    QK_HIP(d->pin_out.ensure(ns * 16 + 64));
+    double* dprice = static_cast<double*>(d->out.p);
+    double* dse = dprice + ns;
+    const int64_t budget = int64_t(std::max(1, env_int("QK_LSMC_MEM_GB", 32))) << 30;
+    QK_HIP(hipEventRecord(d->ev[0], d->stream));
+    QK_HIP(hipEventRecord(d->ev[1], d->stream));
+    size_t first = 0;
+    while (first < all.size()) {
+        // chunk [first, last)
+        size_t last = first;
+        int64_t path_elems = 0, cf_elems = 0, draws = 0;
+        while (last < all.size()) {
+            const int64_t pe = int64_t(all[last].paths) * all[last].steps;
+            const int64_t bytes = (path_elems + pe + cf_elems + all[last].paths) * int64_t(esz);
+            if (last > first && bytes > budget) break;
+            all[last].path_off = path_elems;
+            all[last].cf_off = cf_elems;
+            all[last].stream_off = draws;
+            path_elems += pe;
+            cf_elems += all[last].paths;
+            draws += pe;
+            ++last;
+        }
+        const int32_t count = int32_t(last - first);
+        QK_HIP(d->lsmc_path.ensure(size_t(path_elems) * esz));
+        QK_HIP(d->lsmc_cf.ensure(size_t(cf_elems) * esz));
+        QK_HIP(d->lsmc_recs.ensure(size_t(count) * sizeof(LsmcRec)));
+        QK_HIP(d->pin_in.ensure(size_t(count) * sizeof(LsmcRec)));
+        std::memcpy(d->pin_in.p, all.data() + first, size_t(count) * sizeof(LsmcRec));
+        QK_HIP(hipMemcpyAsync(d->lsmc_recs.p, d->pin_in.p, size_t(count) * sizeof(LsmcRec),
+                              hipMemcpyHostToDevice, d->stream));
+        const void* dstream = nullptr;
+        std::vector<char> host;
+        if (sim_mode != SIM_PHILOX_F32) {
+            // longstaff_schwartz.cpp:61-75: per path, dates 1..N in order
+            host.resize(size_t(draws) * esz);
+            const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::atomic<size_t> next{first};
+            auto work = [&]() {
+                for (;;) {
+                    const size_t k = next.fetch_add(1);
+                    if (k >= last) break;
+                    const LsmcRec& rc = all[k];
+                    std::mt19937_64 eng(seed[rc.out_index]);
+                    std::normal_distribution<double> nd(0.0, 1.0);
+                    const int64_t P = rc.paths, N = rc.steps;
+                    for (int64_t p = 0; p < P; ++p)
+                        for (int64_t j = 0; j < N; ++j) {
+                            const double z = nd(eng);
+                            const size_t at = size_t(rc.stream_off + j * P + p);
+                            if (esz == 8) reinterpret_cast<double*>(host.data())[at] = z;
+                            else reinterpret_cast<float*>(host.data())[at] = float(z);
+                        }
+                }
+            };
+            std::vector<std::thread> pool;
+            for (unsigned k = 1; k < hw; ++k) pool.emplace_back(work);
+            work();
+            for (auto& th : pool) th.join();
+            QK_HIP(d->stream_buf.ensure(host.size()));
+            QK_HIP(hipMemcpyAsync(d->stream_buf.p, host.data(), host.size(),
+                                  hipMemcpyHostToDevice, d->stream));
+            dstream = d->stream_buf.p;
+        }
+        QK_HIP(launch_lsmc(d->stream, sim_mode, basis,
+                           static_cast<const LsmcRec*>(d->lsmc_recs.p), count, d->lsmc_path.p,
+                           d->lsmc_cf.p, dstream, dprice, dse));
+        QK_HIP(hipStreamSynchronize(d->stream));  // pinned recs / host stream reused next chunk
+        first = last;
+    }
+    QK_HIP(hipEventRecord(d->ev[2], d->stream));
+    QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, ns * 16, hipMemcpyDeviceToHost, d->stream));
+    QK_HIP(hipEventRecord(d->ev[3], d->stream));
+    QK_HIP(hipStreamSynchronize(d->stream));
+    record_timing(d);
+    const double* hp = static_cast<const double*>(d->pin_out.p);
+    for (const LsmcRec& rc : all) {
+        out_price[rc.out_index] = hp[rc.out_index];
+        if (out_se) out_se[rc.out_index] = hp[ns + rc.out_index];
+    }
+    return QK_OK;
+}
+
+}  // namespace qk

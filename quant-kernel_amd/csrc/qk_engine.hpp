@@ -1,0 +1,48 @@
+// qk_engine.hpp - host engine behind the C ABI: device context, buffers,
+// validation-mode normal streams and the launch sequence of one MC batch.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+
+#include "qk_kernels.hpp"
+
+namespace qk {
+
+// Host-memory SoA batch (length n).  Null arrays are unused by the method.
+struct HostBatch {
+    const int32_t* model_ids = nullptr;  // QK_MODEL_* per element (mixed batches)
+    int32_t uniform_method = M_HESTON;   // Method when model_ids is null
+    const double *spot = nullptr, *strike = nullptr, *t = nullptr, *vol = nullptr, *r = nullptr,
+                 *q = nullptr;
+    const double *v0 = nullptr, *kappa = nullptr, *theta = nullptr, *sigma = nullptr,
+                 *rho = nullptr;
+    const int32_t *ot = nullptr, *paths = nullptr, *steps = nullptr;
+    const uint64_t* seed = nullptr;
+};
+
+// Runs one batch on the GPU.  Returns a QK_* code; on failure *err names the
+// cause.  out_se may be null.
+int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* out_se,
+                     std::string* err);
+
+// Same with device-resident inputs/outputs (QK_RNG_PHILOX only).
+int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
+                            double* out_se_dev, std::string* err);
+
+// Process-wide RNG mode (QK_RNG_*).
+void set_rng_mode(int32_t mode);
+int32_t get_rng_mode();
+
+// The validation modes' normal stream for one seed (mt19937_64 +
+// std::normal_distribution<double>, as mc_engine.h:15-23).
+void host_normal_stream(uint64_t seed, int64_t count, double* out);
+
+// Event timings of the last completed batch.
+bool last_timing(double* ms_total, double* ms_simulate);
+
+// Device name / CU count; false if no usable device.
+bool device_info(std::string* name, int32_t* cus, std::string* err);
+
+}  // namespace qk

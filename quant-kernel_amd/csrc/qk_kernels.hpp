@@ -1,0 +1,56 @@
+// qk_kernels.hpp - host-side interface of the gfx950 Monte Carlo kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qk_common.hpp"
+
+namespace qk {
+
+// Device-resident SoA inputs of one batch.  Arrays a method does not use may
+// be null.  `method` holds Method values, or QK_MODEL_* ids when
+// method_is_model is set; when `method` is null every element uses
+// uniform_method.
+struct BatchIn {
+    const int32_t* method;
+    const double *spot, *strike, *t, *vol, *r, *q;
+    const double *v0, *kappa, *theta, *sigma, *rho;
+    const int32_t *ot, *paths, *steps;
+    const uint64_t* seed;
+    int32_t uniform_method;
+    int32_t method_is_model;
+};
+
+// Path precision / normal source of a simulation launch.
+enum SimMode : int32_t {
+    SIM_PHILOX_F32 = 0,  // production: Philox + Box-Muller, FP32 paths
+    SIM_STREAM_F64 = 1,  // validation: host normal stream, FP64 paths
+    SIM_STREAM_F32 = 2,  // host normal stream, FP32 paths
+};
+
+// Launch geometry (autotuned per family).
+struct SimShape {
+    int32_t lane_units;  // stepwise: paths per lane per tile; terminal: Philox groups per lane
+    int32_t grid;        // persistent workgroups
+};
+
+constexpr int kBlock = 256;  // threads per workgroup (4 wave64)
+
+inline int32_t tile_units_stepwise(const SimShape& s) { return kBlock * s.lane_units; }
+inline int32_t tile_units_terminal(const SimShape& s) { return kBlock * s.lane_units * 4; }
+
+// counters[0] = next tile, counters[1] = total tiles, counters[2] = overflow flag
+hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev,
+                          const int64_t* stream_off, int32_t tu_terminal, int32_t tu_stepwise,
+                          OptionRec* recs, int32_t* ntiles);
+hipError_t launch_scan(hipStream_t st, const int32_t* n_dev, const int32_t* ntiles,
+                       OptionRec* recs, int32_t* tile_start, int32_t* counters);
+hipError_t launch_simulate(hipStream_t st, int32_t family, int32_t mode, const SimShape& shape,
+                           const OptionRec* recs, const int32_t* tile_start,
+                           const int32_t* n_dev, int32_t* counters, Partial* partials,
+                           const void* stream);
+hipError_t launch_finalize(hipStream_t st, const int32_t* n_dev, const OptionRec* recs,
+                           const Partial* partials, double* out_price, double* out_se);
+
+}  // namespace qk

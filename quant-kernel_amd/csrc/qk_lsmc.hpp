@@ -1,0 +1,38 @@
+// qk_lsmc.hpp - Longstaff-Schwartz (American) batch pricing on gfx950.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace qk {
+
+// One option of an LSMC launch (host-prepared).
+struct alignas(16) LsmcRec {
+    int32_t out_index;  // position in the caller's output arrays
+    int32_t ot, paths, steps;
+    uint32_t seed_lo, seed_hi;
+    int64_t path_off;    // first element of this option's [date][path] matrix
+    int64_t cf_off;      // first element of this option's cashflow vector
+    int64_t stream_off;  // first draw of this option's host stream (validation modes)
+    double spot, strike;
+    double disc;   // e^{-r dt}
+    double drift;  // (r - q - vol^2/2) dt
+    double vs;     // vol sqrt(dt)
+};
+
+// Host entry: validates like longstaff_schwartz.cpp:50-55, resolves the
+// degenerate branch, then simulates on the GPU.  basis is QK_LSMC_BASIS_*.
+int32_t run_lsmc_batch(const double* spot, const double* strike, const double* t,
+                       const double* vol, const double* r, const double* q, const int32_t* ot,
+                       const int32_t* paths, const int32_t* steps, const uint64_t* seed,
+                       int32_t basis, int32_t n, double* out_price, double* out_se,
+                       std::string* err);
+
+// Kernel launcher (qk_lsmc.hip).  mode is a SimMode, basis QK_LSMC_BASIS_*.
+hipError_t launch_lsmc(hipStream_t st, int32_t mode, int32_t basis, const LsmcRec* recs,
+                       int32_t count, void* pathbuf, void* cfbuf, const void* stream,
+                       double* out_price, double* out_se);
+
+}  // namespace qk

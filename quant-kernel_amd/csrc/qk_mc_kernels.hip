@@ -1,0 +1,570 @@
+// qk_mc_kernels.hip - gfx950 kernels of the Monte Carlo batch-pricing path.
+//
+// Pipeline per batch (all on one stream):
+//   prepare  : validate every element and build its OptionRec (degenerate
+//              branches resolved here; the rest get a tile count);
+//   scan     : one workgroup turns tile counts into tile offsets and resets
+//              the persistent tile counter;
+//   simulate : persistent workgroups pull tiles from an atomic counter; a tile
+//              is kBlock lanes x lane_units paths of one option; each lane
+//              simulates whole paths (one path per lane at a time) and the
+//              tile's centred payoff sums are reduced over the wave64 with
+//              shuffles, then over the 4 waves in LDS, in FP64, in a fixed
+//              order;
+//   finalize : per option, a compensated (Neumaier) FP64 sum of its tiles'
+//              partials, discounting and the standard error.
+// Results do not depend on which workgroup ran which tile, so repeated runs
+// are bitwise identical.
+//
+// Reference loops restated: mc_engine.h:32-53 (antithetic terminal),
+// mc_engine.h:84-103 (stepwise), mc_engine.h:113-136 (2-D stepwise),
+// local_vol_monte_carlo.cpp:20-48; step models model_concepts.h:93-266.
+#include "qk_common.hpp"
+#include "qk_kernels.hpp"
+#include "qk_rng.hpp"
+
+namespace qk {
+
+// ----------------------------------------------------------------------------
+// precision helpers
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ float q_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double q_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float q_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ double q_exp2(double x) { return exp2(x); }
+__device__ __forceinline__ float q_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ double q_log2(double x) { return log2(x); }
+template <typename R>
+__device__ __forceinline__ R q_max(R a, R b) { return a > b ? a : b; }
+
+template <typename R>
+__device__ __forceinline__ R payoff(R s, R k, int32_t ot) {
+    const R d = ot == QK_CALL ? s - k : k - s;
+    return d > R(0) ? d : R(0);
+}
+
+// ----------------------------------------------------------------------------
+// normal sources
+// ----------------------------------------------------------------------------
+template <typename R>
+struct Quad {
+    R z[4];
+};
+
+// Production: Philox blocks keyed by the option seed (wave-uniform key).
+struct PhiloxSrc {
+    uint32_t k0, k1;
+    __device__ __forceinline__ Quad<float> path_block(uint32_t p, uint32_t b, int32_t) const {
+        const Normal4 n = philox_normals(p, b, 0u, k0, k1);
+        return Quad<float>{{n.z0, n.z1, n.z2, n.z3}};
+    }
+    __device__ __forceinline__ Quad<float> term_block(uint32_t g) const {
+        const Normal4 n = philox_normals(g, 0u, 1u, k0, k1);
+        return Quad<float>{{n.z0, n.z1, n.z2, n.z3}};
+    }
+    __device__ __forceinline__ float term_draw(uint32_t k) const {
+        const Quad<float> q = term_block(k >> 2);
+        return q.z[k & 3u];
+    }
+};
+
+// Validation: the host's per-option stream laid out draw-major
+// (draw d of path p at off + d*paths + p) so a wave's loads coalesce.
+template <typename R>
+struct StreamSrc {
+    const R* base;
+    int64_t paths;
+    int64_t ndraws;  // draws per path
+    __device__ __forceinline__ Quad<R> path_block(uint32_t p, uint32_t b, int32_t) const {
+        Quad<R> q;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int64_t d = int64_t(b) * 4 + c;
+            q.z[c] = d < ndraws ? base[d * paths + p] : R(0);
+        }
+        return q;
+    }
+    __device__ __forceinline__ Quad<R> term_block(uint32_t g) const {
+        Quad<R> q;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int64_t k = int64_t(g) * 4 + c;
+            q.z[c] = k < ndraws ? base[k] : R(0);
+        }
+        return q;
+    }
+    __device__ __forceinline__ R term_draw(uint32_t k) const { return base[k]; }
+};
+
+// ----------------------------------------------------------------------------
+// prepare / scan
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ int32_t model_to_method(int32_t m) {
+    switch (m) {
+        case QK_MODEL_GBM_EULER: return M_EULER;
+        case QK_MODEL_GBM_MILSTEIN: return M_MILSTEIN;
+        case QK_MODEL_HESTON_EULER: return M_HESTON;
+        case QK_MODEL_LOCAL_VOL: return M_LOCAL_VOL;
+        default: return -1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_prepare(BatchIn in, const int32_t* __restrict__ n_dev,
+                                                 const int64_t* __restrict__ stream_off,
+                                                 int32_t tu_terminal, int32_t tu_stepwise,
+                                                 OptionRec* __restrict__ recs,
+                                                 int32_t* __restrict__ ntiles) {
+    const int32_t n = *n_dev;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        RawOption o;
+        int32_t m = in.method ? in.method[i] : in.uniform_method;
+        if (in.method && in.method_is_model) m = model_to_method(m);
+        o.method = m;
+        o.spot = in.spot[i];
+        o.strike = in.strike[i];
+        o.t = in.t[i];
+        o.vol = in.vol ? in.vol[i] : 0.0;
+        o.r = in.r[i];
+        o.q = in.q[i];
+        o.v0 = in.v0 ? in.v0[i] : 0.0;
+        o.kappa = in.kappa ? in.kappa[i] : 0.0;
+        o.theta = in.theta ? in.theta[i] : 0.0;
+        o.sigma = in.sigma ? in.sigma[i] : 0.0;
+        o.rho = in.rho ? in.rho[i] : 0.0;
+        o.ot = in.ot[i];
+        o.paths = in.paths[i];
+        o.steps = in.steps ? in.steps[i] : 1;
+        o.seed = in.seed[i];
+        OptionRec rec;
+        if (m < 0) {
+            prepare_option(o, 1, rec);  // fills defaults
+            rec.family = FAM_NONE;
+            rec.closed = qnan_d();
+            rec.ntiles = 0;
+        } else {
+            const bool term = (m == M_STANDARD || m == M_ANTITHETIC);
+            prepare_option(o, term ? tu_terminal : tu_stepwise, rec);
+        }
+        if (rec.family == FAM_NONE) rec.ntiles = 0;
+        if (stream_off) rec.stream_off = stream_off[i];
+        recs[i] = rec;
+        ntiles[i] = rec.ntiles;
+    }
+}
+
+// Exclusive scan of the tile counts by one workgroup of 1024 threads; each
+// thread owns a contiguous chunk.  Also resets the tile counter.
+__global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ n_dev,
+                                               const int32_t* __restrict__ ntiles,
+                                               OptionRec* __restrict__ recs,
+                                               int32_t* __restrict__ tile_start,
+                                               int32_t* __restrict__ counters) {
+    __shared__ long long part[1024];
+    const int32_t n = *n_dev;
+    const int32_t tid = threadIdx.x;
+    const int32_t per = (n + 1023) / 1024;
+    const int32_t beg = min(n, tid * per), end = min(n, beg + per);
+    long long local = 0;
+    for (int32_t i = beg; i < end; ++i) local += ntiles[i];
+    part[tid] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    const long long total = part[1023];
+    const bool overflow = total > 0x7fffffffLL;
+    long long run = part[tid] - local;
+    for (int32_t i = beg; i < end; ++i) {
+        const int32_t v = overflow ? 0 : int32_t(run);
+        tile_start[i] = v;
+        recs[i].tile_first = v;
+        run += ntiles[i];
+    }
+    if (tid == 0) {
+        tile_start[n] = overflow ? 0 : int32_t(total);
+        counters[0] = 0;
+        counters[1] = overflow ? 0 : int32_t(total);
+        counters[2] = overflow ? 1 : 0;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// block reduction of (s1, s2, extra) in FP64, fixed order
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void block_store_partial(double s1, double s2, double extra,
+                                                    Partial* out, double (*red)[3]) {
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[wave][0] = s1;
+        red[wave][1] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            a += red[w][0];
+            b += red[w][1];
+        }
+        Partial p;
+        p.s1 = a;
+        p.s2 = b;
+        p.extra = extra;
+        p.pad = 0.0;
+        *out = p;
+    }
+    __syncthreads();
+}
+
+// Largest option index whose first tile is <= tile (tile_start is
+// non-decreasing; empty options share their successor's start).
+__device__ __forceinline__ int32_t find_option(const int32_t* __restrict__ tile_start, int32_t n,
+                                               int32_t tile) {
+    int32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (tile_start[mid] <= tile) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int32_t next_tile(int32_t* counters, int32_t* slot) {
+    if (threadIdx.x == 0) *slot = atomicAdd(&counters[0], 1);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*slot);
+}
+
+// ----------------------------------------------------------------------------
+// path loops
+// ----------------------------------------------------------------------------
+
+// GBM Euler / local vol (Milstein when MIL): s <- max(1e-12, s*(c0 + c1 z + c2 z^2)),
+// the factored form of model_concepts.h:136-158 / 257-266.
+template <typename R, bool MIL, class Src>
+__device__ __forceinline__ R gbm_path(const Src& src, uint32_t p, int32_t steps, R s, R c0, R c1,
+                                      R c2) {
+    const R floor_s = R(1e-12);
+    const int32_t nb = steps >> 2;
+    for (int32_t b = 0; b < nb; ++b) {
+        const Quad<R> z = src.path_block(p, uint32_t(b), steps);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const R f = MIL ? __builtin_fma(__builtin_fma(c2, z.z[c], c1), z.z[c], c0)
+                            : __builtin_fma(c1, z.z[c], c0);
+            s = q_max(floor_s, s * f);
+        }
+    }
+    const int32_t rem = steps & 3;
+    if (rem) {
+        const Quad<R> z = src.path_block(p, uint32_t(nb), steps);
+        for (int c = 0; c < rem; ++c) {
+            const R f = MIL ? __builtin_fma(__builtin_fma(c2, z.z[c], c1), z.z[c], c0)
+                            : __builtin_fma(c1, z.z[c], c0);
+            s = q_max(floor_s, s * f);
+        }
+    }
+    return s;
+}
+
+// Heston full-truncation Euler in x = log2(S/spot) (model_concepts.h:205-226):
+//   x <- max(x + (r-q-v/2)dt log2e + sqrt(v) sqrt(dt) log2e z1, log2(1e-12/spot))
+//   v <- max(0, v + kappa(theta - v)dt + sigma sqrt(v) sqrt(dt) (rho z1 + rho' z2))
+// v >= 0 holds after every step and v0 >= 0 is validated, so max(v,0) = v.
+template <typename R>
+struct HestonC {
+    R a, h, c, kdt, kth, sg, rho, rc, xf, v0;
+};
+
+template <typename R>
+__device__ __forceinline__ void heston_step(const HestonC<R>& k, R& x, R& v, R z1, R z2) {
+    const R sv = q_sqrt(v);
+    const R drift = __builtin_fma(-k.h, v, k.a);
+    x = __builtin_fma(k.c * sv, z1, x + drift);
+    x = q_max(x, k.xf);
+    const R mix = __builtin_fma(k.rho, z1, k.rc * z2);
+    const R vn = __builtin_fma(k.sg * sv, mix, __builtin_fma(-k.kdt, v, v + k.kth));
+    v = q_max(vn, R(0));
+}
+
+template <typename R, class Src>
+__device__ __forceinline__ R heston_path(const Src& src, uint32_t p, int32_t steps,
+                                         const HestonC<R>& k) {
+    R x = R(0), v = k.v0;
+    const int32_t nb = steps >> 1;
+    for (int32_t b = 0; b < nb; ++b) {
+        const Quad<R> z = src.path_block(p, uint32_t(b), steps);
+        heston_step(k, x, v, z.z[0], z.z[1]);
+        heston_step(k, x, v, z.z[2], z.z[3]);
+    }
+    if (steps & 1) {
+        const Quad<R> z = src.path_block(p, uint32_t(nb), steps);
+        heston_step(k, x, v, z.z[0], z.z[1]);
+    }
+    return x;
+}
+
+// ----------------------------------------------------------------------------
+// stepwise simulation (Euler, Milstein, local vol, Heston)
+// ----------------------------------------------------------------------------
+template <typename R, int MODE>
+__global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __restrict__ recs,
+                                                         const int32_t* __restrict__ tile_start,
+                                                         const int32_t* __restrict__ n_dev,
+                                                         int32_t* __restrict__ counters,
+                                                         Partial* __restrict__ partials,
+                                                         const R* __restrict__ stream,
+                                                         int32_t lane_paths) {
+    __shared__ int32_t s_tile;
+    __shared__ double s_red[kBlock / 64][3];
+    const int32_t n = *n_dev;
+    const int32_t total = __builtin_amdgcn_readfirstlane(counters[1]);
+    for (;;) {
+        const int32_t tile = next_tile(counters, &s_tile);
+        if (tile >= total) break;
+        const int32_t opt = find_option(tile_start, n, tile);
+        const OptionRec& rec = recs[opt];
+        const int32_t method = rec.method, steps = rec.steps, paths = rec.paths, ot = rec.ot;
+        const int64_t first = int64_t(tile - rec.tile_first) * kBlock * lane_paths;
+        const R spot = R(rec.spot), strike = R(rec.strike);
+        const double shift = rec.shift;
+
+        PhiloxSrc ph{rec.seed_lo, rec.seed_hi};
+        StreamSrc<R> ss{stream + rec.stream_off, paths,
+                        int64_t(steps) * (method == M_HESTON ? 2 : 1)};
+
+        double s1 = 0.0, s2 = 0.0;
+        if (method == M_HESTON) {
+            HestonC<R> k;
+            k.a = R(rec.c[0]);
+            k.h = R(rec.c[1]);
+            k.c = R(rec.c[2]);
+            k.kdt = R(rec.c[3]);
+            k.kth = R(rec.c[4]);
+            k.sg = R(rec.c[5]);
+            k.rho = R(rec.c[6]);
+            k.rc = R(rec.c[7]);
+            k.xf = R(log2(1e-12 / rec.spot));
+            k.v0 = R(rec.v0);
+            for (int32_t j = 0; j < lane_paths; ++j) {
+                const int64_t p = first + threadIdx.x + int64_t(j) * kBlock;
+                if (p >= paths) break;
+                R x;
+                if constexpr (MODE == SIM_PHILOX_F32) x = heston_path<R>(ph, uint32_t(p), steps, k);
+                else x = heston_path<R>(ss, uint32_t(p), steps, k);
+                const double f = double(payoff(spot * q_exp2(x), strike, ot)) - shift;
+                s1 += f;
+                s2 = __builtin_fma(f, f, s2);
+            }
+        } else {
+            const R c0 = R(rec.c[0]), c1 = R(rec.c[1]), c2 = R(rec.c[2]);
+            const bool mil = method == M_MILSTEIN;
+            for (int32_t j = 0; j < lane_paths; ++j) {
+                const int64_t p = first + threadIdx.x + int64_t(j) * kBlock;
+                if (p >= paths) break;
+                R s;
+                if constexpr (MODE == SIM_PHILOX_F32) {
+                    s = mil ? gbm_path<R, true>(ph, uint32_t(p), steps, spot, c0, c1, c2)
+                            : gbm_path<R, false>(ph, uint32_t(p), steps, spot, c0, c1, c2);
+                } else {
+                    s = mil ? gbm_path<R, true>(ss, uint32_t(p), steps, spot, c0, c1, c2)
+                            : gbm_path<R, false>(ss, uint32_t(p), steps, spot, c0, c1, c2);
+                }
+                const double f = double(payoff(s, strike, ot)) - shift;
+                s1 += f;
+                s2 = __builtin_fma(f, f, s2);
+            }
+        }
+        block_store_partial(s1, s2, 0.0, partials + tile, s_red);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// terminal GBM, antithetic pairs (standard MC and antithetic variates)
+// S_T = spot * 2^(A + B z), A = (r-q-vol^2/2)T log2e, B = vol sqrt(T) log2e.
+// ----------------------------------------------------------------------------
+template <typename R, int MODE>
+__global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __restrict__ recs,
+                                                         const int32_t* __restrict__ tile_start,
+                                                         const int32_t* __restrict__ n_dev,
+                                                         int32_t* __restrict__ counters,
+                                                         Partial* __restrict__ partials,
+                                                         const R* __restrict__ stream,
+                                                         int32_t lane_groups) {
+    __shared__ int32_t s_tile;
+    __shared__ double s_red[kBlock / 64][3];
+    const int32_t n = *n_dev;
+    const int32_t total = __builtin_amdgcn_readfirstlane(counters[1]);
+    for (;;) {
+        const int32_t tile = next_tile(counters, &s_tile);
+        if (tile >= total) break;
+        const int32_t opt = find_option(tile_start, n, tile);
+        const OptionRec& rec = recs[opt];
+        const int32_t pairs = rec.units, ot = rec.ot, paths = rec.paths;
+        const int32_t local_tile = tile - rec.tile_first;
+        const int64_t g0 = int64_t(local_tile) * kBlock * lane_groups;  // first Philox group
+        const R spot = R(rec.spot), strike = R(rec.strike);
+        const R A = R(rec.c[0] * kLog2e), B = R(rec.c[1] * kLog2e);
+        const double shift = rec.shift;
+        PhiloxSrc ph{rec.seed_lo, rec.seed_hi};
+        StreamSrc<R> ss{stream + rec.stream_off, 1, int64_t(pairs) + (paths & 1)};
+
+        double s1 = 0.0, s2 = 0.0;
+        for (int32_t j = 0; j < lane_groups; ++j) {
+            const int64_t g = g0 + threadIdx.x + int64_t(j) * kBlock;
+            if (g * 4 >= pairs) break;
+            Quad<R> z;
+            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_block(uint32_t(g));
+            else z = ss.term_block(uint32_t(g));
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (g * 4 + c < pairs) {
+                    const R e = B * z.z[c];
+                    const R up = payoff(spot * q_exp2(A + e), strike, ot);
+                    const R dn = payoff(spot * q_exp2(A - e), strike, ot);
+                    const double f = 0.5 * (double(up) + double(dn)) - shift;
+                    s1 += f;
+                    s2 = __builtin_fma(f, f, s2);
+                }
+            }
+        }
+        double extra = 0.0;
+        if ((paths & 1) && local_tile == rec.ntiles - 1 && threadIdx.x == 0) {
+            R z;
+            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_draw(uint32_t(pairs));
+            else z = ss.term_draw(uint32_t(pairs));
+            extra = double(payoff(spot * q_exp2(A + B * z), strike, ot));
+        }
+        block_store_partial(s1, s2, extra, partials + tile, s_red);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// finalize
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_finalize(const int32_t* __restrict__ n_dev,
+                                                  const OptionRec* __restrict__ recs,
+                                                  const Partial* __restrict__ partials,
+                                                  double* __restrict__ out_price,
+                                                  double* __restrict__ out_se) {
+    const int32_t n = *n_dev;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const OptionRec& rec = recs[i];
+        if (rec.family == FAM_NONE) {
+            out_price[i] = rec.closed;
+            out_se[i] = rec.closed == rec.closed ? 0.0 : qnan_d();
+            continue;
+        }
+        // Neumaier compensated sums over the option's tiles, in tile order.
+        double a = 0.0, ca = 0.0, b = 0.0, cb = 0.0, extra = 0.0;
+        for (int32_t t = 0; t < rec.ntiles; ++t) {
+            const Partial p = partials[rec.tile_first + t];
+            double s = a + p.s1;
+            ca += fabs(a) >= fabs(p.s1) ? (a - s) + p.s1 : (p.s1 - s) + a;
+            a = s;
+            s = b + p.s2;
+            cb += fabs(b) >= fabs(p.s2) ? (b - s) + p.s2 : (p.s2 - s) + b;
+            b = s;
+            extra += p.extra;
+        }
+        const double s1 = a + ca, s2 = b + cb;
+        const double units = double(rec.units);
+        const double mean_d = s1 / units;
+        double var = units > 1.0 ? (s2 - s1 * mean_d) / (units - 1.0) : 0.0;
+        var = var > 0.0 ? var : 0.0;
+        double price;
+        if (rec.family == FAM_TERMINAL) {
+            const double sum = 2.0 * (rec.shift * units + s1) + extra;
+            price = rec.disc * (sum / double(rec.paths));
+        } else if (rec.method == M_LOCAL_VOL) {
+            const double sum = rec.shift * units + s1;
+            price = rec.disc * sum / double(rec.paths);  // local_vol_monte_carlo.cpp:47
+        } else {
+            price = rec.disc * (rec.shift + mean_d);
+        }
+        out_price[i] = price;
+        out_se[i] = rec.disc * sqrt(var / units);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// host launchers
+// ----------------------------------------------------------------------------
+hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev,
+                          const int64_t* stream_off, int32_t tu_terminal, int32_t tu_stepwise,
+                          OptionRec* recs, int32_t* ntiles) {
+    hipLaunchKernelGGL(k_prepare, dim3(1024), dim3(256), 0, st, in, n_dev, stream_off,
+                       tu_terminal, tu_stepwise, recs, ntiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(hipStream_t st, const int32_t* n_dev, const int32_t* ntiles,
+                       OptionRec* recs, int32_t* tile_start, int32_t* counters) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, n_dev, ntiles, recs, tile_start,
+                       counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_simulate(hipStream_t st, int32_t family, int32_t mode, const SimShape& shape,
+                           const OptionRec* recs, const int32_t* tile_start,
+                           const int32_t* n_dev, int32_t* counters, Partial* partials,
+                           const void* stream) {
+    const dim3 grid(shape.grid), block(kBlock);
+    if (family == FAM_TERMINAL) {
+        switch (mode) {
+            case SIM_PHILOX_F32:
+                hipLaunchKernelGGL((k_sim_terminal<float, SIM_PHILOX_F32>), grid, block, 0, st,
+                                   recs, tile_start, n_dev, counters, partials,
+                                   static_cast<const float*>(stream), shape.lane_units);
+                break;
+            case SIM_STREAM_F64:
+                hipLaunchKernelGGL((k_sim_terminal<double, SIM_STREAM_F64>), grid, block, 0, st,
+                                   recs, tile_start, n_dev, counters, partials,
+                                   static_cast<const double*>(stream), shape.lane_units);
+                break;
+            default:
+                hipLaunchKernelGGL((k_sim_terminal<float, SIM_STREAM_F32>), grid, block, 0, st,
+                                   recs, tile_start, n_dev, counters, partials,
+                                   static_cast<const float*>(stream), shape.lane_units);
+                break;
+        }
+    } else {
+        switch (mode) {
+            case SIM_PHILOX_F32:
+                hipLaunchKernelGGL((k_sim_stepwise<float, SIM_PHILOX_F32>), grid, block, 0, st,
+                                   recs, tile_start, n_dev, counters, partials,
+                                   static_cast<const float*>(stream), shape.lane_units);
+                break;
+            case SIM_STREAM_F64:
+                hipLaunchKernelGGL((k_sim_stepwise<double, SIM_STREAM_F64>), grid, block, 0, st,
+                                   recs, tile_start, n_dev, counters, partials,
+                                   static_cast<const double*>(stream), shape.lane_units);
+                break;
+            default:
+                hipLaunchKernelGGL((k_sim_stepwise<float, SIM_STREAM_F32>), grid, block, 0, st,
+                                   recs, tile_start, n_dev, counters, partials,
+                                   static_cast<const float*>(stream), shape.lane_units);
+                break;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(hipStream_t st, const int32_t* n_dev, const OptionRec* recs,
+                           const Partial* partials, double* out_price, double* out_se) {
+    hipLaunchKernelGGL(k_finalize, dim3(256), dim3(256), 0, st, n_dev, recs, partials, out_price,
+                       out_se);
+    return hipGetLastError();
+}
+
+}  // namespace qk

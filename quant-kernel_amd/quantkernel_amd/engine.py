@@ -1,0 +1,400 @@
+"""Python host interface of the MI355X Monte Carlo batch-pricing library.
+
+Mirrors the Monte Carlo part of the reference's ``QuantKernel`` class
+(python/quantkernel/engine.py:21-41 error classes, :99-124 array coercion,
+:163-194 ``_call_batch_ctypes``, :1352-1823 MC wrappers): the same method
+names, argument order and meaning, NaN for invalid elements, and the same
+rc -> exception mapping.  Every call goes through ``libquantkernel.so`` built
+from ``quant-kernel_amd/csrc`` (HIP kernels for gfx950); there is no CPU
+fallback, so a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import math
+import os
+from pathlib import Path
+
+import numpy as np
+
+QK_OK = 0
+QK_ERR_NULL_PTR = -1
+QK_ERR_BAD_SIZE = -2
+QK_ERR_ABI_MISMATCH = -3
+QK_ERR_RUNTIME_INIT = -4
+QK_ERR_INVALID_INPUT = -5
+QK_CALL = 0
+QK_PUT = 1
+ABI_MAJOR = 2
+ABI_MINOR = 13
+
+QK_RNG_PHILOX = 0
+QK_RNG_MT19937_FP64 = 1
+QK_RNG_MT19937_FP32 = 2
+
+QK_MODEL_GBM_EULER = 0
+QK_MODEL_GBM_MILSTEIN = 1
+QK_MODEL_HESTON_EULER = 2
+QK_MODEL_LOCAL_VOL = 3
+
+QK_LSMC_BASIS_REFERENCE = 0
+QK_LSMC_BASIS_CUBIC = 1
+
+
+class QKError(RuntimeError):
+    """Base exception for QuantKernel errors."""
+
+
+class QKNullPointerError(QKError):
+    """Raised when a null pointer is passed to a batch API."""
+
+
+class QKBadSizeError(QKError):
+    """Raised when an invalid batch size is passed."""
+
+
+class QKInvalidInputError(QKError):
+    """Raised when invalid input parameters are detected."""
+
+
+class QKRuntimeInitError(QKError):
+    """Raised when no usable gfx950 device is available."""
+
+
+_ERROR_CODE_MAP = {
+    QK_ERR_NULL_PTR: QKNullPointerError,
+    QK_ERR_BAD_SIZE: QKBadSizeError,
+    QK_ERR_INVALID_INPUT: QKInvalidInputError,
+    QK_ERR_RUNTIME_INIT: QKRuntimeInitError,
+}
+
+
+class QKPluginAPI(ct.Structure):
+    _fields_ = [("abi_major", ct.c_int32), ("abi_minor", ct.c_int32), ("plugin_name", ct.c_char_p)]
+
+
+_D = ct.c_double
+_I = ct.c_int32
+_U = ct.c_uint64
+_PD = ct.POINTER(ct.c_double)
+_PI = ct.POINTER(ct.c_int32)
+_PU = ct.POINTER(ct.c_uint64)
+
+# Every exported symbol and its signature (restype, argtypes).
+_SIGNATURES = {
+    "qk_abi_version": (None, [_PI, _PI]),
+    "qk_plugin_get_api": (_I, [_I, _I, ct.POINTER(ct.POINTER(QKPluginAPI))]),
+    "qk_get_last_error": (ct.c_char_p, []),
+    "qk_clear_last_error": (None, []),
+    "qk_mcm_standard_monte_carlo_price": (_D, [_D] * 6 + [_I, _I, _U]),
+    "qk_mcm_standard_monte_carlo_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_standard_monte_carlo_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_mcm_antithetic_variates_price": (_D, [_D] * 6 + [_I, _I, _U]),
+    "qk_mcm_antithetic_variates_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_euler_maruyama_price": (_D, [_D] * 6 + [_I, _I, _I, _U]),
+    "qk_mcm_euler_maruyama_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_euler_maruyama_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_mcm_milstein_price": (_D, [_D] * 6 + [_I, _I, _I, _U]),
+    "qk_mcm_milstein_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_milstein_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_mcm_local_vol_monte_carlo_price": (_D, [_D] * 6 + [_I, _I, _I, _U]),
+    "qk_mcm_local_vol_monte_carlo_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_local_vol_monte_carlo_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_mcm_longstaff_schwartz_price": (_D, [_D] * 6 + [_I, _I, _I, _U]),
+    "qk_mcm_longstaff_schwartz_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_longstaff_schwartz_price_batch_ex": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PU, _I, _I, _PD, _PD]),
+    "qk_mcm_heston_monte_carlo_price": (_D, [_D] * 10 + [_I, _I, _I, _U]),
+    "qk_mcm_heston_monte_carlo_price_batch": (_I, [_PD] * 10 + [_PI, _PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_heston_monte_carlo_price_batch_se": (_I, [_PD] * 10 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_mcm_mixed_price_batch": (_I, [_PI] + [_PD] * 11 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_mcm_mixed_price_batch_device": (_I, [ct.c_void_p] * 16 + [_I, ct.c_void_p, ct.c_void_p]),
+    "qk_mc_set_rng_mode": (_I, [_I]),
+    "qk_mc_get_rng_mode": (_I, []),
+    "qk_mc_last_timing": (_I, [_PD, _PD]),
+    "qk_mc_validation_stream": (_I, [_U, ct.c_int64, _PD]),
+    "qk_device_info": (_I, [ct.c_char_p, _I, _PI]),
+}
+
+
+def _default_lib_path() -> Path:
+    env = os.environ.get("QK_LIB_PATH")
+    if env:
+        p = Path(env)
+        return p / "libquantkernel.so" if p.is_dir() else p
+    return Path(__file__).resolve().parent.parent / "libquantkernel.so"
+
+
+def load_library(path: str | os.PathLike | None = None) -> ct.CDLL:
+    """Load libquantkernel.so and declare every signature; raises if absent."""
+    p = Path(path) if path is not None else _default_lib_path()
+    if not p.exists():
+        raise QKRuntimeInitError(
+            f"libquantkernel.so not found at {p}; build it with __graft_entry__.build()"
+        )
+    lib = ct.CDLL(str(p))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)  # AttributeError if a symbol is missing
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+class QuantKernel:
+    """Monte Carlo batch pricing on MI355X behind the QuantKernel C ABI."""
+
+    CALL = QK_CALL
+    PUT = QK_PUT
+
+    def __init__(self, lib_path: str | os.PathLike | None = None):
+        self._lib = load_library(lib_path)
+        major, minor = self.abi_version()
+        if major != ABI_MAJOR or minor < 12:
+            raise QKError(f"ABI mismatch: library {major}.{minor}")
+
+    # ---- ABI / errors (engine.py:130-161) --------------------------------
+    def abi_version(self) -> tuple[int, int]:
+        major, minor = ct.c_int32(-1), ct.c_int32(-1)
+        self._lib.qk_abi_version(ct.byref(major), ct.byref(minor))
+        return int(major.value), int(minor.value)
+
+    def plugin_get_api(self, host_abi_major: int = ABI_MAJOR, host_abi_minor: int = ABI_MINOR):
+        out = ct.POINTER(QKPluginAPI)()
+        rc = self._lib.qk_plugin_get_api(int(host_abi_major), int(host_abi_minor), ct.byref(out))
+        if rc != QK_OK:
+            raise QKError(f"qk_plugin_get_api failed (rc={rc}, host_abi={host_abi_major}.{host_abi_minor})")
+        info = out.contents
+        return int(info.abi_major), int(info.abi_minor), info.plugin_name.decode()
+
+    def get_last_error(self) -> str:
+        raw = self._lib.qk_get_last_error()
+        return raw.decode("utf-8", errors="replace") if raw else ""
+
+    def clear_last_error(self) -> None:
+        self._lib.qk_clear_last_error()
+
+    # ---- mode / device ---------------------------------------------------
+    def set_rng_mode(self, mode: int) -> None:
+        self._check(self._lib.qk_mc_set_rng_mode(int(mode)), "qk_mc_set_rng_mode")
+
+    def get_rng_mode(self) -> int:
+        return int(self._lib.qk_mc_get_rng_mode())
+
+    def last_timing(self) -> tuple[float, float]:
+        a, b = ct.c_double(), ct.c_double()
+        self._check(self._lib.qk_mc_last_timing(ct.byref(a), ct.byref(b)), "qk_mc_last_timing")
+        return a.value, b.value
+
+    def device_info(self) -> tuple[str, int]:
+        buf = ct.create_string_buffer(256)
+        cus = ct.c_int32()
+        self._check(self._lib.qk_device_info(buf, 256, ct.byref(cus)), "qk_device_info")
+        return buf.value.decode(), int(cus.value)
+
+    def validation_stream(self, seed: int, count: int) -> np.ndarray:
+        out = np.empty(int(count), dtype=np.float64)
+        self._check(self._lib.qk_mc_validation_stream(int(seed), int(count), out.ctypes.data_as(_PD)),
+                    "qk_mc_validation_stream")
+        return out
+
+    # ---- helpers (engine.py:99-124, 163-194) -----------------------------
+    @staticmethod
+    def _as_f64_array(x, name: str) -> np.ndarray:
+        arr = np.asarray(x, dtype=np.float64)
+        if arr.ndim != 1:
+            raise ValueError(f"{name} must be a 1D array-like")
+        return np.ascontiguousarray(arr)
+
+    @staticmethod
+    def _as_i32_array(x, name: str) -> np.ndarray:
+        arr = np.asarray(x, dtype=np.int32)
+        if arr.ndim != 1:
+            raise ValueError(f"{name} must be a 1D array-like")
+        return np.ascontiguousarray(arr)
+
+    @staticmethod
+    def _as_u64_array(x, name: str) -> np.ndarray:
+        arr = np.asarray(x, dtype=np.uint64)
+        if arr.ndim != 1:
+            raise ValueError(f"{name} must be a 1D array-like")
+        return np.ascontiguousarray(arr)
+
+    @staticmethod
+    def _check_same_length(n: int, **arrays) -> None:
+        for name, arr in arrays.items():
+            if arr.shape[0] != n:
+                raise ValueError(f"{name} length mismatch: expected {n}, got {arr.shape[0]}")
+
+    def _check(self, rc: int, fn_name: str) -> None:
+        if rc != QK_OK:
+            detail = self.get_last_error()
+            exc = _ERROR_CODE_MAP.get(rc, QKError)
+            msg = f"{fn_name} failed (rc={rc})"
+            if detail:
+                msg += f": {detail}"
+            raise exc(msg)
+
+    @staticmethod
+    def _ptr(arr: np.ndarray):
+        if arr.dtype == np.float64:
+            return arr.ctypes.data_as(_PD)
+        if arr.dtype == np.int32:
+            return arr.ctypes.data_as(_PI)
+        if arr.dtype == np.uint64:
+            return arr.ctypes.data_as(_PU)
+        raise TypeError(f"Unsupported dtype for batch call: {arr.dtype}")
+
+    def _call_batch(self, fn_name: str, *arrays, extra_int=None, with_se=False):
+        n = arrays[0].shape[0]
+        out = np.empty(n, dtype=np.float64)
+        se = np.empty(n, dtype=np.float64) if with_se else None
+        args = [self._ptr(a) for a in arrays]
+        if extra_int is not None:
+            args.append(int(extra_int))
+        args.append(int(n))
+        args.append(out.ctypes.data_as(_PD))
+        if with_se:
+            args.append(se.ctypes.data_as(_PD))
+        rc = getattr(self._lib, fn_name)(*args)
+        self._check(rc, fn_name)
+        return (out, se) if with_se else out
+
+    def _gbm_arrays(self, spot, strike, t, vol, r, q, option_type):
+        spot = self._as_f64_array(spot, "spot")
+        n = spot.shape[0]
+        arrs = dict(
+            strike=self._as_f64_array(strike, "strike"), t=self._as_f64_array(t, "t"),
+            vol=self._as_f64_array(vol, "vol"), r=self._as_f64_array(r, "r"),
+            q=self._as_f64_array(q, "q"), option_type=self._as_i32_array(option_type, "option_type"),
+        )
+        self._check_same_length(n, **arrs)
+        return [spot] + list(arrs.values())
+
+    # ---- standard / antithetic (engine.py:1352-1393, 1466-1483, 1616-1647) -
+    def standard_monte_carlo_price(self, spot, strike, t, vol, r, q, option_type, paths=100000, seed=42):
+        return float(self._lib.qk_mcm_standard_monte_carlo_price(
+            spot, strike, t, vol, r, q, int(option_type), int(paths), int(seed)))
+
+    def standard_monte_carlo_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, seed,
+                                         return_stderr=False):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        p, s = self._as_i32_array(paths, "paths"), self._as_u64_array(seed, "seed")
+        self._check_same_length(a[0].shape[0], paths=p, seed=s)
+        if return_stderr:
+            return self._call_batch("qk_mcm_standard_monte_carlo_price_batch_se", *a, p, s, with_se=True)
+        return self._call_batch("qk_mcm_standard_monte_carlo_price_batch", *a, p, s)
+
+    def antithetic_variates_price(self, spot, strike, t, vol, r, q, option_type, paths=100000, seed=42):
+        return float(self._lib.qk_mcm_antithetic_variates_price(
+            spot, strike, t, vol, r, q, int(option_type), int(paths), int(seed)))
+
+    def antithetic_variates_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, seed):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        p, s = self._as_i32_array(paths, "paths"), self._as_u64_array(seed, "seed")
+        self._check_same_length(a[0].shape[0], paths=p, seed=s)
+        return self._call_batch("qk_mcm_antithetic_variates_price_batch", *a, p, s)
+
+    # ---- stepwise GBM (engine.py:1394-1411, 1484-1517, 1777-1823) ---------
+    def _stepwise_scalar(self, fn, spot, strike, t, vol, r, q, option_type, paths, steps, seed):
+        return float(getattr(self._lib, fn)(spot, strike, t, vol, r, q, int(option_type), int(paths),
+                                            int(steps), int(seed)))
+
+    def _stepwise_batch(self, fn, spot, strike, t, vol, r, q, option_type, paths, steps, seed, se):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        p, st, s = (self._as_i32_array(paths, "paths"), self._as_i32_array(steps, "steps"),
+                    self._as_u64_array(seed, "seed"))
+        self._check_same_length(a[0].shape[0], paths=p, steps=st, seed=s)
+        if se:
+            return self._call_batch(fn + "_se", *a, p, st, s, with_se=True)
+        return self._call_batch(fn, *a, p, st, s)
+
+    def euler_maruyama_price(self, spot, strike, t, vol, r, q, option_type, paths=10000, steps=100, seed=42):
+        return self._stepwise_scalar("qk_mcm_euler_maruyama_price", spot, strike, t, vol, r, q, option_type,
+                                     paths, steps, seed)
+
+    def euler_maruyama_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, steps, seed,
+                                   return_stderr=False):
+        return self._stepwise_batch("qk_mcm_euler_maruyama_price_batch", spot, strike, t, vol, r, q,
+                                    option_type, paths, steps, seed, return_stderr)
+
+    def milstein_price(self, spot, strike, t, vol, r, q, option_type, paths=10000, steps=100, seed=42):
+        return self._stepwise_scalar("qk_mcm_milstein_price", spot, strike, t, vol, r, q, option_type,
+                                     paths, steps, seed)
+
+    def milstein_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, steps, seed,
+                             return_stderr=False):
+        return self._stepwise_batch("qk_mcm_milstein_price_batch", spot, strike, t, vol, r, q,
+                                    option_type, paths, steps, seed, return_stderr)
+
+    def local_vol_monte_carlo_price(self, spot, strike, t, vol, r, q, option_type, paths=50000, steps=200,
+                                    seed=42):
+        return self._stepwise_scalar("qk_mcm_local_vol_monte_carlo_price", spot, strike, t, vol, r, q,
+                                     option_type, paths, steps, seed)
+
+    def local_vol_monte_carlo_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, steps, seed,
+                                          return_stderr=False):
+        return self._stepwise_batch("qk_mcm_local_vol_monte_carlo_price_batch", spot, strike, t, vol, r, q,
+                                    option_type, paths, steps, seed, return_stderr)
+
+    # ---- LSMC (engine.py:1412-1420, 1518-1534) ---------------------------
+    def longstaff_schwartz_price(self, spot, strike, t, vol, r, q, option_type, paths=10000, steps=50,
+                                 seed=42):
+        return self._stepwise_scalar("qk_mcm_longstaff_schwartz_price", spot, strike, t, vol, r, q,
+                                     option_type, paths, steps, seed)
+
+    def longstaff_schwartz_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, steps, seed,
+                                       basis=None, return_stderr=False):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        p, st, s = (self._as_i32_array(paths, "paths"), self._as_i32_array(steps, "steps"),
+                    self._as_u64_array(seed, "seed"))
+        self._check_same_length(a[0].shape[0], paths=p, steps=st, seed=s)
+        if basis is None and not return_stderr:
+            return self._call_batch("qk_mcm_longstaff_schwartz_price_batch", *a, p, st, s)
+        out, se = self._call_batch("qk_mcm_longstaff_schwartz_price_batch_ex", *a, p, st, s,
+                                   extra_int=QK_LSMC_BASIS_REFERENCE if basis is None else basis, with_se=True)
+        return (out, se) if return_stderr else out
+
+    # ---- Heston (engine.py:1648-1718) ------------------------------------
+    def heston_monte_carlo_price(self, spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type,
+                                 paths=50000, steps=100, seed=42):
+        return float(self._lib.qk_mcm_heston_monte_carlo_price(
+            spot, strike, t, r, q, v0, kappa, theta, sigma, rho, int(option_type), int(paths), int(steps),
+            int(seed)))
+
+    def heston_monte_carlo_price_batch(self, spot, strike, t, r, q, v0, kappa, theta, sigma, rho,
+                                       option_type, paths, steps, seed, return_stderr=False):
+        f = [self._as_f64_array(x, nm) for x, nm in
+             zip((spot, strike, t, r, q, v0, kappa, theta, sigma, rho),
+                 ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho"))]
+        n = f[0].shape[0]
+        ot, p, st, s = (self._as_i32_array(option_type, "option_type"), self._as_i32_array(paths, "paths"),
+                        self._as_i32_array(steps, "steps"), self._as_u64_array(seed, "seed"))
+        self._check_same_length(n, **{nm: x for nm, x in zip(
+            ("strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho"), f[1:])},
+            option_type=ot, paths=p, steps=st, seed=s)
+        if return_stderr:
+            return self._call_batch("qk_mcm_heston_monte_carlo_price_batch_se", *f, ot, p, st, s, with_se=True)
+        return self._call_batch("qk_mcm_heston_monte_carlo_price_batch", *f, ot, p, st, s)
+
+    # ---- mixed-model batch (ABI 2.13) ------------------------------------
+    def mixed_price_batch(self, model, spot, strike, t, vol, r, q, v0, kappa, theta, sigma, rho, option_type,
+                          paths, steps, seed):
+        m = self._as_i32_array(model, "model")
+        f = [self._as_f64_array(x, nm) for x, nm in
+             zip((spot, strike, t, vol, r, q, v0, kappa, theta, sigma, rho),
+                 ("spot", "strike", "t", "vol", "r", "q", "v0", "kappa", "theta", "sigma", "rho"))]
+        ot, p, st, s = (self._as_i32_array(option_type, "option_type"), self._as_i32_array(paths, "paths"),
+                        self._as_i32_array(steps, "steps"), self._as_u64_array(seed, "seed"))
+        n = m.shape[0]
+        self._check_same_length(n, spot=f[0], option_type=ot, paths=p, steps=st, seed=s)
+        return self._call_batch("qk_mcm_mixed_price_batch", m, *f, ot, p, st, s, with_se=True)
+
+    def mixed_price_batch_device(self, n: int, ptrs: dict, out_price_ptr: int, out_se_ptr: int) -> None:
+        """Device-resident variant: ``ptrs`` maps argument names to device addresses."""
+        order = ("model", "spot", "strike", "t", "vol", "r", "q", "v0", "kappa", "theta", "sigma", "rho",
+                 "option_type", "paths", "steps", "seed")
+        args = [ct.c_void_p(int(ptrs[k])) for k in order]
+        rc = self._lib.qk_mcm_mixed_price_batch_device(*args, int(n), ct.c_void_p(int(out_price_ptr)),
+                                                       ct.c_void_p(int(out_se_ptr)))
+        self._check(rc, "qk_mcm_mixed_price_batch_device")
+
+

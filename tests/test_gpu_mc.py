@@ -1,0 +1,283 @@
+"""GPU parity of the MC batch path against the oracle and the reference.
+
+Two normal sources are checked:
+  * validation modes feed the reference's own mt19937_64 stream; with FP64
+    paths the GPU must match the reference within 1e-6 relative (the
+    reference goldens of cpp/tests/test_monte_carlo.cpp:57-73 included);
+  * Philox mode (production, FP32 paths) must equal the FP64 oracle fed the
+    same Philox stream to FP32 rounding, and agree with the reference's
+    mt19937 results within 3 standard errors.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from quantkernel_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+RNG_PHILOX, RNG_F64, RNG_F32 = 0, 1, 2
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+# --------------------------------------------------------------------------
+# validation mode (mt19937 stream, FP64 paths) == reference
+# --------------------------------------------------------------------------
+def test_goldens_validation_mode(qk):
+    qk.set_rng_mode(RNG_F64)
+    c = qk.standard_monte_carlo_price(100.0, 100.0, 1.0, 0.2, 0.05, 0.0, 0, 50000, 12345)
+    p = qk.standard_monte_carlo_price(100.0, 100.0, 1.0, 0.2, 0.05, 0.0, 1, 50000, 12345)
+    e = qk.euler_maruyama_price(100.0, 100.0, 1.0, 0.2, 0.05, 0.0, 0, 10000, 100, 12345)
+    print("goldens", repr(c), repr(p), repr(e))
+    assert rel(c, 10.533371511490282) < 1e-6
+    assert rel(p, 5.6296577522043556) < 1e-6
+    assert rel(e, 10.218359279533905) < 1e-6
+
+
+def _small_gbm(n, seed0=7):
+    d = wl.config0(n, paths=2001)
+    d["steps"] = (np.arange(n) % 7 + 3).astype(np.int32)
+    d["seed"] = (np.arange(n) + seed0).astype(np.uint64)
+    return d
+
+
+@pytest.mark.parametrize("method", ["standard", "antithetic", "euler", "milstein", "local_vol"])
+def test_validation_fp64_matches_oracle(qk, orc, method):
+    qk.set_rng_mode(RNG_F64)
+    d = _small_gbm(12)
+    args = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths")]
+    if method in ("standard", "antithetic"):
+        fn = getattr(qk, f"{method}_monte_carlo_price_batch" if method == "standard"
+                     else "antithetic_variates_price_batch")
+        got = fn(*args, d["seed"])
+        ofn = orc.lib.orc_standard_mc if method == "standard" else orc.lib.orc_antithetic
+        want = [ofn(*[float(a[i]) if a.dtype.kind == "f" else int(a[i]) for a in args], int(d["seed"][i]))
+                for i in range(12)]
+    else:
+        name = {"euler": "euler_maruyama", "milstein": "milstein", "local_vol": "local_vol_monte_carlo"}[method]
+        got = getattr(qk, f"{name}_price_batch")(*args, d["steps"], d["seed"])
+        ofn = {"euler": orc.lib.orc_euler, "milstein": orc.lib.orc_milstein,
+               "local_vol": orc.lib.orc_local_vol}[method]
+        want = [ofn(*[float(a[i]) if a.dtype.kind == "f" else int(a[i]) for a in args], int(d["steps"][i]),
+                    int(d["seed"][i])) for i in range(12)]
+    worst = max(rel(g, w) for g, w in zip(got, want))
+    print(method, "worst rel", worst)
+    assert worst < 1e-6
+
+
+def test_heston_validation_fp64_matches_oracle(qk, orc):
+    qk.set_rng_mode(RNG_F64)
+    d = wl.config1(10, paths=1500, steps=37)
+    keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
+    d["option_type"] = (np.arange(10) % 2).astype(np.int32)
+    got = qk.heston_monte_carlo_price_batch(*[d[k] for k in keys], d["option_type"], d["paths"], d["steps"],
+                                            d["seed"])
+    want = [orc.lib.orc_heston(*[float(d[k][i]) for k in keys], int(d["option_type"][i]), int(d["paths"][i]),
+                               int(d["steps"][i]), int(d["seed"][i])) for i in range(10)]
+    worst = max(rel(g, w) for g, w in zip(got, want))
+    print("heston worst rel", worst)
+    assert worst < 1e-6
+
+
+def test_validation_matches_reference_build(qk, ref):
+    """Same check against the reference library itself (when it was built)."""
+    qk.set_rng_mode(RNG_F64)
+    h = qk.heston_monte_carlo_price(100.0, 95.0, 0.7, 0.03, 0.01, 0.05, 1.7, 0.06, 0.6, -0.6, 0, 3000, 50, 99)
+    r = ref.mcm_heston_monte_carlo_price(100.0, 95.0, 0.7, 0.03, 0.01, 0.05, 1.7, 0.06, 0.6, -0.6, 0, 3000, 50,
+                                         99)
+    m = qk.milstein_price(90.0, 100.0, 1.3, 0.35, 0.02, 0.01, 1, 2000, 40, 5)
+    rm = ref.mcm_milstein_price(90.0, 100.0, 1.3, 0.35, 0.02, 0.01, 1, 2000, 40, 5)
+    print("heston", h, r, "milstein", m, rm)
+    assert rel(h, r) < 1e-6 and rel(m, rm) < 1e-6
+
+
+def test_validation_fp32_paths_deviation(qk, orc):
+    """FP32 path arithmetic on the reference stream: reported, loosely bounded."""
+    qk.set_rng_mode(RNG_F32)
+    d = wl.config1(6, paths=4000, steps=64)
+    keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
+    got = qk.heston_monte_carlo_price_batch(*[d[k] for k in keys], d["option_type"], d["paths"], d["steps"],
+                                            d["seed"])
+    want = [orc.lib.orc_heston(*[float(d[k][i]) for k in keys], 0, 4000, 64, int(d["seed"][i]))
+            for i in range(6)]
+    dev = [abs(g - w) / d["spot"][i] for i, (g, w) in enumerate(zip(got, want))]
+    print("fp32-path deviation (per unit spot)", max(dev))
+    assert max(dev) < 1e-4
+
+
+# --------------------------------------------------------------------------
+# Philox mode == FP64 oracle on the same stream
+# --------------------------------------------------------------------------
+def test_philox_heston_matches_fp64_oracle(qk, orc):
+    d = wl.config1(8, paths=3000, steps=33)
+    keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
+    d["option_type"] = (np.arange(8) % 2).astype(np.int32)
+    got, se = qk.heston_monte_carlo_price_batch(*[d[k] for k in keys], d["option_type"], d["paths"],
+                                                d["steps"], d["seed"], return_stderr=True)
+    for i in range(8):
+        w, wse = orc.px("heston", *[float(d[k][i]) for k in keys], int(d["option_type"][i]), 3000, 33,
+                        int(d["seed"][i]))
+        print(i, got[i], w, se[i], wse)
+        # FP32 path error per unit spot, and the SE estimate itself
+        assert abs(got[i] - w) / d["spot"][i] < 2e-5
+        assert abs(se[i] - wse) <= 1e-3 * wse + 1e-9
+
+
+@pytest.mark.parametrize("model", [0, 1, 3])
+def test_philox_stepwise_matches_fp64_oracle(qk, orc, model):
+    d = _small_gbm(8)
+    args = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths")]
+    name = {0: "euler_maruyama", 1: "milstein", 3: "local_vol_monte_carlo"}[model]
+    got, se = getattr(qk, f"{name}_price_batch")(*args, d["steps"], d["seed"], return_stderr=True)
+    for i in range(8):
+        w, wse = orc.px("stepwise", model, *[float(a[i]) if a.dtype.kind == "f" else int(a[i]) for a in args],
+                        int(d["steps"][i]), int(d["seed"][i]))
+        assert abs(got[i] - w) / d["spot"][i] < 2e-5, (i, got[i], w)
+        assert abs(se[i] - wse) <= 1e-3 * wse + 1e-9
+
+
+def test_philox_terminal_matches_fp64_oracle(qk, orc):
+    d = wl.config0(16, paths=5001)
+    args = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths")]
+    got, se = qk.standard_monte_carlo_price_batch(*args, d["seed"], return_stderr=True)
+    for i in range(16):
+        w, wse = orc.px("terminal", *[float(a[i]) if a.dtype.kind == "f" else int(a[i]) for a in args],
+                        int(d["seed"][i]), 1)
+        assert abs(got[i] - w) / d["spot"][i] < 2e-6, (i, got[i], w)
+        assert abs(se[i] - wse) <= 1e-3 * wse + 1e-9
+
+
+# --------------------------------------------------------------------------
+# Philox mode vs the reference (mt19937) and closed forms: statistical
+# --------------------------------------------------------------------------
+def test_philox_vs_reference_within_3se(qk, orc):
+    d = wl.config1(24, paths=20000, steps=64)
+    keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
+    got, se = qk.heston_monte_carlo_price_batch(*[d[k] for k in keys], d["option_type"], d["paths"],
+                                                d["steps"], d["seed"], return_stderr=True)
+    qk.set_rng_mode(RNG_F64)
+    ref_like = qk.heston_monte_carlo_price_batch(*[d[k] for k in keys], d["option_type"], d["paths"],
+                                                 d["steps"], d["seed"])
+    z = (got - ref_like) / (se * math.sqrt(2.0))  # two independent estimates
+    print("z", np.round(z, 2))
+    assert np.all(np.abs(z) < 4.5)
+    assert abs(np.mean(z)) < 3.0 / math.sqrt(24) * 1.5
+
+
+def test_terminal_vs_bsm_within_3se(qk, orc):
+    d = wl.config0(64, paths=200_000)
+    args = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths")]
+    got, se = qk.standard_monte_carlo_price_batch(*args, d["seed"], return_stderr=True)
+    bsm = np.array([orc.lib.orc_bsm(*[float(a[i]) if a.dtype.kind == "f" else int(a[i]) for a in args[:7]])
+                    for i in range(64)])
+    z = (got - bsm) / se
+    print("max |z|", np.max(np.abs(z)))
+    assert np.max(np.abs(z)) < 4.5
+
+
+def test_heston_vs_characteristic_function(qk, orc):
+    d = wl.config1(12, paths=200_000, steps=256)
+    keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
+    got, se = qk.heston_monte_carlo_price_batch(*[d[k] for k in keys], d["option_type"], d["paths"],
+                                                d["steps"], d["seed"], return_stderr=True)
+    cf = np.array([orc.lib.orc_heston_cf(*[float(d[k][i]) for k in keys], 0, 4096, 200.0) for i in range(12)])
+    z = (got - cf) / se
+    print("heston z vs CF", np.round(z, 2), "rel", np.round((got - cf) / cf, 4))
+    # Euler discretisation bias at 256 steps is small but not zero: 4 SE + 0.5% of the price
+    assert np.all(np.abs(got - cf) < 4 * se + 5e-3 * np.abs(cf) + 1e-3)
+
+
+# --------------------------------------------------------------------------
+# ABI semantics on the device path
+# --------------------------------------------------------------------------
+def test_degenerate_and_invalid_elements(qk, orc):
+    nan = float("nan")
+    spot = np.array([100, 100, 100, 100, 100, nan, 100, 100, -1.0])
+    strike = np.array([90, 110, 100, 100, 100, 100, 100, 100, 100.0])
+    t = np.array([0.0, 1e-13, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0])
+    vol = np.array([0.2, 0.2, 0.0, 0.2, 0.2, 0.2, 0.2, 0.2, 0.2])
+    r = np.full(9, 0.05)
+    q = np.full(9, 0.01)
+    ot = np.array([0, 1, 0, 0, 7, 0, 0, 1, 0], dtype=np.int32)
+    paths = np.array([1000, 1000, 1000, 1, 1000, 1000, 3, 2, 1000], dtype=np.int32)
+    seed = np.arange(9, dtype=np.uint64)
+    got = qk.standard_monte_carlo_price_batch(spot, strike, t, vol, r, q, ot, paths, seed)
+    want = [orc.lib.orc_standard_mc(float(spot[i]), float(strike[i]), float(t[i]), float(vol[i]), 0.05, 0.01,
+                                    int(ot[i]), int(paths[i]), int(seed[i])) for i in range(9)]
+    for g, w in zip(got[:4], want[:4]):
+        assert (math.isnan(g) and math.isnan(w)) or g == pytest.approx(w, rel=1e-12)
+    assert math.isnan(got[3]) and math.isnan(got[4]) and math.isnan(got[5]) and math.isnan(got[8])
+    assert got[6] > 0 and np.isfinite(got[7])
+    # antithetic variates has no vol <= eps branch: vol = 0 is simulated
+    a = qk.antithetic_variates_price_batch(spot[2:3], strike[2:3], t[2:3], vol[2:3], r[:1], q[:1], ot[2:3],
+                                           paths[2:3], seed[2:3])
+    assert a[0] == pytest.approx(orc.lib.orc_antithetic(100, 100, 1, 0, .05, .01, 0, 1000, 2), rel=1e-5)
+
+
+def test_batch_equals_scalar_and_deterministic(qk):
+    d = wl.config1(5, paths=5000, steps=20)
+    keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
+    b1 = qk.heston_monte_carlo_price_batch(*[d[k] for k in keys], d["option_type"], d["paths"], d["steps"],
+                                           d["seed"])
+    b2 = qk.heston_monte_carlo_price_batch(*[d[k] for k in keys], d["option_type"], d["paths"], d["steps"],
+                                           d["seed"])
+    assert np.array_equal(b1, b2)
+    for i in range(5):
+        s = qk.heston_monte_carlo_price(*[float(d[k][i]) for k in keys], 0, 5000, 20, int(d["seed"][i]))
+        assert s == b1[i]
+
+
+def test_mixed_equals_single_model(qk):
+    d = wl.config2(40, p_lo=2000, p_hi=20000, s_lo=4, s_hi=40)
+    price, se = qk.mixed_price_batch(d["model"], d["spot"], d["strike"], d["t"], d["vol"], d["r"], d["q"],
+                                     d["v0"], d["kappa"], d["theta"], d["sigma"], d["rho"], d["option_type"],
+                                     d["paths"], d["steps"], d["seed"])
+    for i in range(40):
+        m = int(d["model"][i])
+        if m == 2:
+            s = qk.heston_monte_carlo_price(d["spot"][i], d["strike"][i], d["t"][i], d["r"][i], d["q"][i],
+                                            d["v0"][i], d["kappa"][i], d["theta"][i], d["sigma"][i], d["rho"][i],
+                                            int(d["option_type"][i]), int(d["paths"][i]), int(d["steps"][i]),
+                                            int(d["seed"][i]))
+        else:
+            fn = {0: qk.euler_maruyama_price, 1: qk.milstein_price, 3: qk.local_vol_monte_carlo_price}[m]
+            s = fn(d["spot"][i], d["strike"][i], d["t"][i], d["vol"][i], d["r"][i], d["q"][i],
+                   int(d["option_type"][i]), int(d["paths"][i]), int(d["steps"][i]), int(d["seed"][i]))
+        assert s == price[i], (i, m, s, price[i])
+        assert se[i] > 0
+
+
+# --------------------------------------------------------------------------
+# Longstaff-Schwartz
+# --------------------------------------------------------------------------
+def test_lsmc_validation_matches_oracle(qk, orc):
+    qk.set_rng_mode(RNG_F64)
+    d = wl.config3(6, paths=3000, steps=20)
+    args = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths", "steps")]
+    got = qk.longstaff_schwartz_price_batch(*args, d["seed"])
+    want = [orc.lib.orc_lsmc(*[float(a[i]) if a.dtype.kind == "f" else int(a[i]) for a in args],
+                             int(d["seed"][i]), 0) for i in range(6)]
+    worst = max(rel(g, w) for g, w in zip(got, want))
+    print("lsmc worst", worst, got, want)
+    assert worst < 1e-6
+    gotc = qk.longstaff_schwartz_price_batch(*args, d["seed"], basis=1)
+    wantc = [orc.lib.orc_lsmc(*[float(a[i]) if a.dtype.kind == "f" else int(a[i]) for a in args],
+                              int(d["seed"][i]), 1) for i in range(6)]
+    assert max(rel(g, w) for g, w in zip(gotc, wantc)) < 1e-6
+
+
+def test_lsmc_philox_matches_fp64_oracle_and_crr(qk, orc):
+    d = wl.config3(4, paths=20000, steps=50)
+    args = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths", "steps")]
+    got, se = qk.longstaff_schwartz_price_batch(*args, d["seed"], basis=1, return_stderr=True)
+    for i in range(4):
+        a = [float(x[i]) if x.dtype.kind == "f" else int(x[i]) for x in args]
+        w, wse = orc.px("lsmc", *a, int(d["seed"][i]), 1)
+        crr = orc.lib.orc_crr(*a[:7], 2000, 1)
+        print(i, got[i], w, se[i], crr)
+        assert abs(got[i] - w) < 0.25 * se[i] + 1e-4 * a[0]
+        assert abs(got[i] - crr) < 4 * se[i] + 0.02
