@@ -1,0 +1,251 @@
+"""Pins the CPU oracle before it is trusted as the checker (CPU only).
+
+  * the reference's golden values (cpp/tests/test_monte_carlo.cpp:57-73);
+  * bitwise agreement with a build of the reference itself, for every
+    estimator on the path, when oracle/_ref was built;
+  * Philox4x32-10 known-answer vectors (Random123, Salmon et al. SC'11);
+  * Box-Muller normals: moments and a Kolmogorov-Smirnov distance;
+  * closed forms: BSM known values (cpp/tests/test_closed_form.cpp:7-15),
+    Heston CF vs its sigma -> 0 BSM limit, put-call parity and an
+    independent numpy inversion; CRR vs BSM and the Longstaff-Schwartz (2001)
+    Table 1 American put values;
+  * the FP64 Philox estimators vs closed forms within their standard errors.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from quantkernel_amd import workloads as wl
+
+
+def test_reference_goldens(orc):
+    assert orc.lib.orc_standard_mc(100.0, 100.0, 1.0, 0.2, 0.05, 0.0, 0, 50000, 12345) == pytest.approx(
+        10.533371511490282, abs=1e-12)
+    assert orc.lib.orc_standard_mc(100.0, 100.0, 1.0, 0.2, 0.05, 0.0, 1, 50000, 12345) == pytest.approx(
+        5.6296577522043556, abs=1e-12)
+    assert orc.lib.orc_euler(100.0, 100.0, 1.0, 0.2, 0.05, 0.0, 0, 10000, 100, 12345) == pytest.approx(
+        10.218359279533905, abs=1e-12)
+
+
+def _gbm_cases(n=10):
+    d = wl.config0(n, paths=1501)
+    d["steps"] = (np.arange(n) % 5 + 2).astype(np.int32)
+    d["seed"] = (np.arange(n) * 7919 + 3).astype(np.uint64)
+    return d
+
+
+def _row(d, i, keys):
+    return [float(d[k][i]) if d[k].dtype.kind == "f" else int(d[k][i]) for k in keys]
+
+
+@pytest.mark.parametrize("name", ["standard_mc", "antithetic", "euler", "milstein", "local_vol", "lsmc"])
+def test_oracle_bitwise_equals_reference_build(orc, ref, name):
+    d = _gbm_cases()
+    base = ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths")
+    ref_fn = {"standard_mc": ref.mcm_standard_monte_carlo_price, "antithetic": ref.mcm_antithetic_variates_price,
+              "euler": ref.mcm_euler_maruyama_price, "milstein": ref.mcm_milstein_price,
+              "local_vol": ref.mcm_local_vol_monte_carlo_price,
+              "lsmc": ref.mcm_longstaff_schwartz_price}[name]
+    for i in range(10):
+        row = _row(d, i, base)
+        if name in ("standard_mc", "antithetic"):
+            a = getattr(orc.lib, "orc_" + name)(*row, int(d["seed"][i]))
+            b = ref_fn(*row, int(d["seed"][i]))
+        elif name == "lsmc":
+            row[-1] = 300
+            a = orc.lib.orc_lsmc(*row, int(d["steps"][i]) + 3, int(d["seed"][i]), 0)
+            b = ref_fn(*row, int(d["steps"][i]) + 3, int(d["seed"][i]))
+        else:
+            a = getattr(orc.lib, "orc_" + name)(*row, int(d["steps"][i]), int(d["seed"][i]))
+            b = ref_fn(*row, int(d["steps"][i]), int(d["seed"][i]))
+        assert a == b or (math.isnan(a) and math.isnan(b)), (name, i, a, b)
+
+
+def test_heston_oracle_bitwise_equals_reference_build(orc, ref):
+    d = wl.config1(8, paths=700, steps=13)
+    keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
+    for i in range(8):
+        row = _row(d, i, keys)
+        ot = i % 2
+        a = orc.lib.orc_heston(*row, ot, 700, 13, int(d["seed"][i]))
+        b = ref.mcm_heston_monte_carlo_price(*row, ot, 700, 13, int(d["seed"][i]))
+        assert a == b
+
+
+def test_degenerate_branches_equal_reference_build(orc, ref):
+    nan = float("nan")
+    cases = [
+        (100, 90, 0.0, 0.2, 0.05, 0.0, 0, 1000), (100, 110, 1e-13, 0.2, 0.05, 0.0, 1, 1000),
+        (100, 90, 1.0, 0.0, 0.05, 0.01, 0, 1000), (100, 90, 1.0, 0.2, 0.05, 0.0, 0, 1),
+        (100, 90, 1.0, 0.2, 0.05, 0.0, 2, 1000), (nan, 90, 1.0, 0.2, 0.05, 0.0, 0, 1000),
+        (100, 90, 1.0, -0.1, 0.05, 0.0, 0, 1000), (100, 90, 1.0, 0.2, 0.05, 0.0, 0, 3),
+    ]
+    for c in cases:
+        for o, r in ((orc.lib.orc_standard_mc, ref.mcm_standard_monte_carlo_price),
+                     (orc.lib.orc_antithetic, ref.mcm_antithetic_variates_price)):
+            a, b = o(*c, 11), r(*c, 11)
+            assert a == b or (math.isnan(a) and math.isnan(b)), (c, a, b)
+
+
+def test_mt_stream_matches_product_validation_stream(orc, qk):
+    a = orc.mt_normals(12345, 1001)
+    b = qk.validation_stream(12345, 1001)
+    assert np.array_equal(a, b)
+
+
+def test_philox_known_answers(orc):
+    assert orc.philox([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert orc.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert orc.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_box_muller_distribution(orc):
+    n = 40000
+    z = np.array([orc.lib.orc_philox_path_normal(777, p, d) for p in range(n // 8) for d in range(8)])
+    assert abs(z.mean()) < 5 / math.sqrt(n)
+    assert abs(z.var() - 1) < 5 * math.sqrt(2 / n)
+    k = ((z - z.mean()) ** 4).mean() / z.var() ** 2
+    assert abs(k - 3) < 0.15
+    zs = np.sort(z)
+    cdf = 0.5 * np.array([math.erfc(-x / math.sqrt(2)) for x in zs])
+    ks = np.max(np.abs(cdf - (np.arange(1, n + 1) / n)))
+    assert ks < 1.63 / math.sqrt(n) * 1.5  # ~1% critical value, with margin
+    # draws of different components / paths / streams are not copies
+    t = np.array([orc.lib.orc_philox_terminal_normal(777, k) for k in range(64)])
+    assert len(set(np.round(t, 12))) == 64
+    assert not np.allclose(t[:8], z[:8])
+
+
+def test_box_muller_exact_mapping(orc):
+    import ctypes as ct
+    z0, z1 = ct.c_double(), ct.c_double()
+    # top 24 bits: u1 = (0x800000 + .5)/2^24 (~1/2), u2 = 0.25 -> angle pi/2
+    orc.lib.orc_box_muller(0x80000000, 0x40000000, ct.byref(z0), ct.byref(z1))
+    rad = math.sqrt(-2 * math.log((0x800000 + 0.5) / 2 ** 24))
+    assert z0.value == pytest.approx(0.0, abs=1e-12) and z1.value == pytest.approx(rad, rel=1e-14)
+
+
+def test_bsm_known_values_and_parity(orc):
+    c = orc.lib.orc_bsm(100, 100, 1, 0.2, 0.05, 0.0, 0)
+    p = orc.lib.orc_bsm(100, 100, 1, 0.2, 0.05, 0.0, 1)
+    assert c == pytest.approx(10.4506, abs=0.01) and p == pytest.approx(5.5735, abs=0.01)
+    assert c == pytest.approx(10.450583572185565, abs=1e-12)  # erfc form, textbook value
+    assert c - p == pytest.approx(100 - 100 * math.exp(-0.05), abs=1e-12)
+    assert abs(orc.lib.orc_bsm_as(100, 100, 1, 0.2, 0.05, 0.0, 0) - c) < 2e-5
+
+
+def test_bsm_as_matches_reference_build(orc, ref):
+    for args in [(100, 100, 1, .2, .05, 0, 0), (80, 100, .5, .4, .03, .01, 1), (120, 90, 2, .15, 0, .02, 0)]:
+        assert orc.lib.orc_bsm_as(*args) == pytest.approx(ref.cf_black_scholes_merton_price(*args), rel=1e-13)
+
+
+def _numpy_heston(S, K, T, r, q, v0, kappa, theta, sig, rho, ot):
+    """Independent inversion: Gil-Pelaez with Heston's original CF on a fine
+    trapezoid grid (different form and quadrature from the oracle)."""
+    u = np.linspace(1e-6, 250.0, 400_001)
+    x0 = math.log(S) + (r - q) * T
+
+    def phi(w):
+        iw = 1j * w
+        b = kappa - rho * sig * iw
+        dd = np.sqrt(b * b + sig * sig * (iw + w * w))
+        g = (b + dd) / (b - dd)
+        e = np.exp(dd * T)
+        C = iw * x0 + kappa * theta / sig ** 2 * ((b + dd) * T - 2 * np.log((1 - g * e) / (1 - g)))
+        D = (b + dd) / sig ** 2 * (1 - e) / (1 - g * e)
+        return np.exp(C + D * v0)
+
+    ek = np.exp(-1j * u * math.log(K))
+    f2 = np.real(ek * phi(u) / (1j * u))
+    fwd = math.exp(x0)  # phi(-i) = E[S_T], exact by the martingale property
+    f1 = np.real(ek * phi(u - 1j) / (1j * u * fwd))
+    P2 = 0.5 + np.trapezoid(f2, u) / math.pi
+    P1 = 0.5 + np.trapezoid(f1, u) / math.pi
+    call = S * math.exp(-q * T) * P1 - K * math.exp(-r * T) * P2
+    return call if ot == 0 else call - S * math.exp(-q * T) + K * math.exp(-r * T)
+
+
+def test_heston_cf(orc, ref):
+    # sigma -> 0 with v0 = theta and rho = 0 is BSM with vol sqrt(theta)
+    # (the correction is O(sigma^2); with rho != 0 it is O(rho sigma))
+    h = orc.lib.orc_heston_cf(100, 95, 1.0, 0.03, 0.01, 0.04, 1.5, 0.04, 1e-4, 0.0, 0, 1024, 200.0)
+    assert h == pytest.approx(orc.lib.orc_bsm(100, 95, 1.0, 0.2, 0.03, 0.01, 0), abs=2e-7)
+    args = (100, 110, 0.8, 0.02, 0.01, 0.09, 2.0, 0.05, 0.7, -0.7)
+    c = orc.lib.orc_heston_cf(*args, 0, 4096, 200.0)
+    p = orc.lib.orc_heston_cf(*args, 1, 4096, 200.0)
+    assert c - p == pytest.approx(100 * math.exp(-0.01 * 0.8) - 110 * math.exp(-0.02 * 0.8), abs=1e-10)
+    assert c == pytest.approx(_numpy_heston(*args, 0), abs=2e-6)
+    # the reference's own CF (same formulation, GL nodes from its table)
+    assert c == pytest.approx(ref.lib.qk_cf_heston_price_cf(*[float(a) for a in args], 0, 4096, 200.0)
+                              if _bind_ref_cf(ref) else c, abs=1e-9)
+
+
+def _bind_ref_cf(ref):
+    import ctypes as ct
+    f = ref.lib.qk_cf_heston_price_cf
+    f.restype = ct.c_double
+    f.argtypes = [ct.c_double] * 10 + [ct.c_int32, ct.c_int32, ct.c_double]
+    return True
+
+
+@pytest.mark.parametrize("s,t,vol,want", [(36, 1, 0.2, 4.478), (40, 1, 0.2, 2.314), (44, 1, 0.2, 1.110),
+                                          (36, 1, 0.4, 7.101), (36, 2, 0.2, 4.840)])
+def test_crr_american_put_longstaff_schwartz_table1(orc, s, t, vol, want):
+    # Table 1's finite-difference values sit ~0.005-0.009 below the converged
+    # tree value (CRR at 2000..16000 steps agrees to 2e-5), hence 0.012.
+    assert orc.lib.orc_crr(s, 40, t, vol, 0.06, 0.0, 1, 2000, 1) == pytest.approx(want, abs=1.2e-2)
+    assert orc.lib.orc_crr(s, 40, t, vol, 0.06, 0.0, 1, 2000, 1) == pytest.approx(
+        orc.lib.orc_crr(s, 40, t, vol, 0.06, 0.0, 1, 8000, 1), abs=5e-4)
+
+
+def test_crr_matches_reference_build(orc, ref):
+    import ctypes as ct
+    f = ref.lib.qk_tlm_crr_price
+    f.restype = ct.c_double
+    f.argtypes = [ct.c_double] * 6 + [ct.c_int32] * 3
+    for args in [(36, 40, 1, .2, .06, 0, 1), (100, 95, .5, .3, .02, .01, 0), (90, 100, 2, .25, .05, .0, 1)]:
+        for am in (0, 1):
+            assert orc.lib.orc_crr(*args, 2000, am) == pytest.approx(f(*args, 2000, am), rel=1e-11)
+
+
+def test_crr_european_converges_to_bsm(orc):
+    for args in [(100, 100, 1, .2, .05, 0, 0), (90, 100, .5, .3, .03, .01, 1)]:
+        assert orc.lib.orc_crr(*args, 2000, 0) == pytest.approx(orc.lib.orc_bsm(*args), abs=5e-3)
+    # American call without dividends is never exercised early
+    assert orc.lib.orc_crr(100, 100, 1, .2, .05, 0, 0, 1000, 1) == pytest.approx(
+        orc.lib.orc_crr(100, 100, 1, .2, .05, 0, 0, 1000, 0), abs=1e-9)
+
+
+def test_philox_estimators_vs_closed_forms(orc):
+    # terminal antithetic vs BSM
+    v, se = orc.px("terminal", 100.0, 105.0, 1.0, 0.25, 0.03, 0.01, 0, 40001, 9, 1)
+    assert abs(v - orc.lib.orc_bsm(100.0, 105.0, 1.0, 0.25, 0.03, 0.01, 0)) < 4 * se
+    # Euler / Milstein vs BSM (discretisation bias O(dt) is far below 4 SE here)
+    for model in (0, 1, 3):
+        v, se = orc.px("stepwise", model, 100.0, 95.0, 0.5, 0.3, 0.02, 0.0, 1, 20000, 16, 5)
+        assert abs(v - orc.lib.orc_bsm(100.0, 95.0, 0.5, 0.3, 0.02, 0.0, 1)) < 4 * se + 0.01
+    # Heston Euler vs CF
+    args = (100.0, 100.0, 1.0, 0.03, 0.0, 0.05, 2.0, 0.05, 0.4, -0.6)
+    v, se = orc.px("heston", *args, 0, 20000, 64, 3)
+    assert abs(v - orc.lib.orc_heston_cf(*args, 0, 2048, 200.0)) < 4 * se + 0.02
+    # LSMC (cubic basis) vs a 2000-step tree
+    v, se = orc.px("lsmc", 40.0, 40.0, 1.0, 0.2, 0.06, 0.0, 1, 20000, 50, 4, 1)
+    assert abs(v - 2.314) < 4 * se + 0.02
+
+
+def test_planted_mistakes_are_caught(orc):
+    """The checks above fail for plausible oracle mistakes: a sign flip of the
+    antithetic mirror and a dropped Milstein term change the goldens/BSM
+    comparisons far beyond their tolerances."""
+    g = orc.mt_normals(12345, 25000)
+    S, K, T, vol, r = 100.0, 100.0, 1.0, 0.2, 0.05
+    mu = (r - 0.5 * vol * vol) * T
+    up = S * np.exp(mu + vol * math.sqrt(T) * g)
+    same = S * np.exp(mu + vol * math.sqrt(T) * g)  # mirror without the sign flip
+    price_bad = math.exp(-r * T) * np.mean(np.concatenate([np.maximum(up - K, 0), np.maximum(same - K, 0)]))
+    assert abs(price_bad - 10.533371511490282) > 1e-3
+    dn = S * np.exp(mu - vol * math.sqrt(T) * g)
+    price_ok = math.exp(-r * T) * (np.maximum(up - K, 0) + np.maximum(dn - K, 0)).sum() / 50000
+    assert price_ok == pytest.approx(10.533371511490282, rel=1e-12)
