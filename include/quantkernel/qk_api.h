@@ -204,6 +204,12 @@ QK_EXPORT int32_t qk_mc_last_timing(double* ms_total, double* ms_simulate);
  * only; needs no device. */
 QK_EXPORT int32_t qk_mc_validation_stream(uint64_t seed, int64_t count, double* out);
 
+/* Self-check of the device normal generator: the Philox4x32-10 words and
+ * the FP32 Box-Muller normals of blocks (c0 + i, c1, 0, c3), i < count, keyed
+ * by seed (4 words and 4 normals per block). */
+QK_EXPORT int32_t qk_mc_philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3,
+                                     int32_t count, uint32_t* out_words, float* out_normals);
+
 /* Name of the device in use and its compute-unit count; QK_ERR_RUNTIME_INIT
  * if there is none. */
 QK_EXPORT int32_t qk_device_info(char* name_buf, int32_t buf_len, int32_t* compute_units);

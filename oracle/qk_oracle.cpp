@@ -109,14 +109,15 @@ void philox10(const uint32_t in[4], const uint32_t key[2], uint32_t out[4]) {
     out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
 }
 
-// Box-Muller (1958) on two 32-bit words, exactly the uniforms the GPU uses:
-// u1 = (top 24 bits + 1/2) / 2^24 in (0,1), u2 = top 24 bits / 2^24 in [0,1);
-// (z0, z1) = sqrt(-2 ln u1) * (cos 2 pi u2, sin 2 pi u2).
+// Box-Muller (1958) on two 32-bit words, with exactly the uniforms the GPU
+// forms: u1 = fmaf(float(a), 2^-32, 2^-33) in FP32 (in (0, 1]) and the angle
+// fraction f = (b & 0x7FFFFF) / 2^23; (z0, z1) = sqrt(-2 ln u1) *
+// (cos 2 pi f, sin 2 pi f), evaluated here in FP64.
 void box_muller(uint32_t a, uint32_t b, double* z0, double* z1) {
-    const double u1 = (double(a >> 8) + 0.5) * (1.0 / 16777216.0);
-    const double u2 = double(b >> 8) * (1.0 / 16777216.0);
+    const double u1 = static_cast<double>(std::fmaf(static_cast<float>(a), 0x1p-32f, 0x1p-33f));
+    const double f = static_cast<double>(b & 0x007FFFFFu) * (1.0 / 8388608.0);
     const double rad = std::sqrt(-2.0 * std::log(u1));
-    const double ang = 6.283185307179586476925286766559 * u2;
+    const double ang = 6.283185307179586476925286766559 * f;
     *z0 = rad * std::cos(ang);
     *z1 = rad * std::sin(ang);
 }

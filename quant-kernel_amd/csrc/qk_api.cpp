@@ -392,6 +392,16 @@ int32_t qk_device_info(char* name_buf, int32_t buf_len, int32_t* compute_units) 
     return QK_OK;
 }
 
+int32_t qk_mc_philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3, int32_t count,
+                           uint32_t* out_words, float* out_normals) {
+    if (!check_ptrs({QK_P(out_words), QK_P(out_normals)})) return QK_ERR_NULL_PTR;
+    if (!check_n(count)) return QK_ERR_BAD_SIZE;
+    std::string err;
+    const int32_t rc = qk::philox_probe(seed, c0, c1, c3, count, out_words, out_normals, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
 int32_t qk_mc_validation_stream(uint64_t seed, int64_t count, double* out) {
     if (!check_ptrs({QK_P(out)})) return QK_ERR_NULL_PTR;
     if (count < 0) {

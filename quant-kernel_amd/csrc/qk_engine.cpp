@@ -240,6 +240,25 @@ void host_normal_stream(uint64_t seed, int64_t count, double* out) {
     for (int64_t i = 0; i < count; ++i) out[i] = nd(eng);
 }
 
+int32_t philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3, int32_t count,
+                     uint32_t* words, float* normals, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+    const size_t bytes = size_t(count) * 16;
+    QK_HIP(d->out.ensure(2 * bytes));
+    QK_HIP(d->pin_out.ensure(2 * bytes));
+    uint32_t* dw = static_cast<uint32_t*>(d->out.p);
+    float* dn = reinterpret_cast<float*>(static_cast<char*>(d->out.p) + bytes);
+    QK_HIP(launch_philox_probe(d->stream, seed, c0, c1, c3, count, dw, dn));
+    QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, 2 * bytes, hipMemcpyDeviceToHost, d->stream));
+    QK_HIP(hipStreamSynchronize(d->stream));
+    std::memcpy(words, d->pin_out.p, bytes);
+    std::memcpy(normals, static_cast<char*>(d->pin_out.p) + bytes, bytes);
+    return QK_OK;
+}
+
 bool last_timing(double* ms_total, double* ms_simulate) {
     if (!g_dev || g_dev->ms_total < 0.0) return false;
     if (ms_total) *ms_total = g_dev->ms_total;

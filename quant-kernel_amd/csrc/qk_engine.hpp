@@ -39,6 +39,10 @@ int32_t get_rng_mode();
 // std::normal_distribution<double>, as mc_engine.h:15-23).
 void host_normal_stream(uint64_t seed, int64_t count, double* out);
 
+// Device Philox words and FP32 normals of blocks (c0+i, c1, 0, c3), i < count.
+int32_t philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3, int32_t count,
+                     uint32_t* words, float* normals, std::string* err);
+
 // Event timings of the last completed batch.
 bool last_timing(double* ms_total, double* ms_simulate);
 

@@ -53,4 +53,7 @@ hipError_t launch_simulate(hipStream_t st, int32_t family, int32_t mode, const S
 hipError_t launch_finalize(hipStream_t st, const int32_t* n_dev, const OptionRec* recs,
                            const Partial* partials, double* out_price, double* out_se);
 
+hipError_t launch_philox_probe(hipStream_t st, uint64_t seed, uint32_t c0, uint32_t c1,
+                               uint32_t c3, int32_t count, uint32_t* words, float* normals);
+
 }  // namespace qk

@@ -34,8 +34,12 @@ __device__ __forceinline__ float q_exp2(float x) { return __builtin_amdgcn_exp2f
 __device__ __forceinline__ double q_exp2(double x) { return exp2(x); }
 __device__ __forceinline__ float q_log2(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ double q_log2(double x) { return log2(x); }
-template <typename R>
-__device__ __forceinline__ R q_max(R a, R b) { return a > b ? a : b; }
+__device__ __forceinline__ float q_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double q_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// Inputs are validated finite, so IEEE maxNum and the reference's
+// std::max agree on every value that can occur here.
+__device__ __forceinline__ float q_max(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double q_max(double a, double b) { return __builtin_fmax(a, b); }
 
 template <typename R>
 __device__ __forceinline__ R payoff(R s, R k, int32_t ot) {
@@ -260,8 +264,7 @@ __device__ __forceinline__ R gbm_path(const Src& src, uint32_t p, int32_t steps,
         const Quad<R> z = src.path_block(p, uint32_t(b), steps);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const R f = MIL ? __builtin_fma(__builtin_fma(c2, z.z[c], c1), z.z[c], c0)
-                            : __builtin_fma(c1, z.z[c], c0);
+            const R f = MIL ? q_fma(q_fma(c2, z.z[c], c1), z.z[c], c0) : q_fma(c1, z.z[c], c0);
             s = q_max(floor_s, s * f);
         }
     }
@@ -269,8 +272,7 @@ __device__ __forceinline__ R gbm_path(const Src& src, uint32_t p, int32_t steps,
     if (rem) {
         const Quad<R> z = src.path_block(p, uint32_t(nb), steps);
         for (int c = 0; c < rem; ++c) {
-            const R f = MIL ? __builtin_fma(__builtin_fma(c2, z.z[c], c1), z.z[c], c0)
-                            : __builtin_fma(c1, z.z[c], c0);
+            const R f = MIL ? q_fma(q_fma(c2, z.z[c], c1), z.z[c], c0) : q_fma(c1, z.z[c], c0);
             s = q_max(floor_s, s * f);
         }
     }
@@ -289,11 +291,11 @@ struct HestonC {
 template <typename R>
 __device__ __forceinline__ void heston_step(const HestonC<R>& k, R& x, R& v, R z1, R z2) {
     const R sv = q_sqrt(v);
-    const R drift = __builtin_fma(-k.h, v, k.a);
-    x = __builtin_fma(k.c * sv, z1, x + drift);
+    const R drift = q_fma(-k.h, v, k.a);
+    x = q_fma(k.c * sv, z1, x + drift);
     x = q_max(x, k.xf);
-    const R mix = __builtin_fma(k.rho, z1, k.rc * z2);
-    const R vn = __builtin_fma(k.sg * sv, mix, __builtin_fma(-k.kdt, v, v + k.kth));
+    const R mix = q_fma(k.rho, z1, k.rc * z2);
+    const R vn = q_fma(k.sg * sv, mix, q_fma(-k.kdt, v, v + k.kth));
     v = q_max(vn, R(0));
 }
 
@@ -495,6 +497,37 @@ __global__ __launch_bounds__(256) void k_finalize(const int32_t* __restrict__ n_
         out_price[i] = price;
         out_se[i] = rec.disc * sqrt(var / units);
     }
+}
+
+// ----------------------------------------------------------------------------
+// stream self-check: raw Philox words and FP32 normals of blocks
+// (c0 + i, c1, 0, c3), i < count
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_philox_probe(uint32_t k0, uint32_t k1, uint32_t c0,
+                                                      uint32_t c1, uint32_t c3, int32_t count,
+                                                      uint32_t* __restrict__ words,
+                                                      float* __restrict__ normals) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const U4 w = philox4x32_10(c0 + uint32_t(i), c1, c3, k0, k1);
+    words[4 * i + 0] = w.x;
+    words[4 * i + 1] = w.y;
+    words[4 * i + 2] = w.z;
+    words[4 * i + 3] = w.w;
+    float a, b, c, d;
+    box_muller_f32(w.x, w.y, a, b);
+    box_muller_f32(w.z, w.w, c, d);
+    normals[4 * i + 0] = a;
+    normals[4 * i + 1] = b;
+    normals[4 * i + 2] = c;
+    normals[4 * i + 3] = d;
+}
+
+hipError_t launch_philox_probe(hipStream_t st, uint64_t seed, uint32_t c0, uint32_t c1,
+                               uint32_t c3, int32_t count, uint32_t* words, float* normals) {
+    hipLaunchKernelGGL(k_philox_probe, dim3((count + 255) / 256), dim3(256), 0, st,
+                       uint32_t(seed), uint32_t(seed >> 32), c0, c1, c3, count, words, normals);
+    return hipGetLastError();
 }
 
 // ----------------------------------------------------------------------------

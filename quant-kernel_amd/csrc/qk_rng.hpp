@@ -4,9 +4,11 @@
 // by the option's 64-bit seed, followed by Box-Muller (1958) in FP32 on the
 // hardware transcendental units.  Draw d of path p of an option is component
 // d%4 of block (p, d/4, 0, 0); draw k of a terminal (antithetic) estimator is
-// component k%4 of block (k/4, 0, 0, 1).  Components: words (0,1) give
-// (r cos, r sin) and words (2,3) the next pair, with
-// u1 = (w>>8 + 1/2)/2^24, u2 = (w>>8)/2^24 and r = sqrt(-2 ln u1).
+// component k%4 of block (k/4, 0, 0, 1).  Words (w0,w1) give components 0,1
+// and (w2,w3) components 2,3 of one block:
+//   u1 = fma(float(w_even), 2^-32, 2^-33)           (FP32, in (0, 1])
+//   f  = (w_odd & 0x7FFFFF) / 2^23                   (exact, in [0, 1))
+//   (z_even, z_odd) = sqrt(-2 ln u1) * (cos 2 pi f, sin 2 pi f)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,33 +20,68 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
-// One Philox4x32-10 block.  The key words are wave-uniform in every caller,
-// so the Weyl key schedule runs on the scalar unit.
-__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
+
+// a ^ b ^ s in one gfx950 v_bitop3_b32 (truth table 0x96); s is wave-uniform.
+__device__ __forceinline__ uint32_t xor3_vvs(uint32_t a, uint32_t b, uint32_t s) {
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "s"(s));
+    return r;
+}
+
+__device__ __forceinline__ void mulhilo(uint32_t m, uint32_t x, uint32_t& hi, uint32_t& lo) {
+    const uint64_t p = uint64_t(m) * x;
+    hi = uint32_t(p >> 32);
+    lo = uint32_t(p);
+}
+
+// Philox4x32-10 of the counter (c0, c1, 0, c3) where c0 varies per lane and
+// c1, c3, k0, k1 are wave-uniform.  Rounds 1-3 are left to the compiler so
+// the uniform parts stay on the scalar unit; rounds 4-10 use the 3-input xor.
+__device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c3, uint32_t k0,
+                                            uint32_t k1) {
+    uint32_t x0 = c0, x1 = c1, x2 = 0u, x3 = c3;
 #pragma unroll
-    for (int round = 0; round < 10; ++round) {
-        const uint64_t pa = uint64_t(0xD2511F53u) * c.x;
-        const uint64_t pb = uint64_t(0xCD9E8D57u) * c.z;
-        const uint32_t nx = uint32_t(pb >> 32) ^ c.y ^ k0;
-        const uint32_t nz = uint32_t(pa >> 32) ^ c.w ^ k1;
-        c.y = uint32_t(pb);
-        c.w = uint32_t(pa);
-        c.x = nx;
-        c.z = nz;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
+    for (int round = 0; round < 3; ++round) {
+        uint32_t h0, l0, h2, l2;
+        mulhilo(kPhiloxM0, x0, h0, l0);
+        mulhilo(kPhiloxM1, x2, h2, l2);
+        const uint32_t n0 = h2 ^ x1 ^ k0;
+        const uint32_t n2 = h0 ^ x3 ^ k1;
+        x0 = n0;
+        x1 = l2;
+        x2 = n2;
+        x3 = l0;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
     }
-    return c;
+#pragma unroll
+    for (int round = 3; round < 10; ++round) {
+        uint32_t h0, l0, h2, l2;
+        mulhilo(kPhiloxM0, x0, h0, l0);
+        mulhilo(kPhiloxM1, x2, h2, l2);
+        const uint32_t n0 = xor3_vvs(h2, x1, k0);
+        const uint32_t n2 = xor3_vvs(h0, x3, k1);
+        x0 = n0;
+        x1 = l2;
+        x2 = n2;
+        x3 = l0;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    return U4{x0, x1, x2, x3};
 }
 
 // Box-Muller in FP32: v_log_f32 (log2), v_sqrt_f32, and v_sin/v_cos, which
-// take their argument in revolutions, so u2 is used as is.
+// take their argument in revolutions: 1 + f in [1, 2) has the same sine and
+// cosine as f and is built with one and-or.
 __device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float& z0, float& z1) {
-    const float u1 = __builtin_fmaf(float(a >> 8), 0x1p-24f, 0x1p-25f);
-    const float u2 = float(b >> 8) * 0x1p-24f;
+    const float u1 = __builtin_fmaf(float(a), 0x1p-32f, 0x1p-33f);
+    const float rev = __builtin_bit_cast(float, (b & 0x007FFFFFu) | 0x3F800000u);
     const float rad = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u1) * -1.3862943611198906f);
-    z0 = rad * __builtin_amdgcn_cosf(u2);
-    z1 = rad * __builtin_amdgcn_sinf(u2);
+    z0 = rad * __builtin_amdgcn_cosf(rev);
+    z1 = rad * __builtin_amdgcn_sinf(rev);
 }
 
 struct Normal4 {
@@ -53,7 +90,7 @@ struct Normal4 {
 
 __device__ __forceinline__ Normal4 philox_normals(uint32_t c0, uint32_t c1, uint32_t c3,
                                                   uint32_t k0, uint32_t k1) {
-    const U4 w = philox4x32_10(U4{c0, c1, 0u, c3}, k0, k1);
+    const U4 w = philox4x32_10(c0, c1, c3, k0, k1);
     Normal4 n;
     box_muller_f32(w.x, w.y, n.z0, n.z1);
     box_muller_f32(w.z, w.w, n.z2, n.z3);

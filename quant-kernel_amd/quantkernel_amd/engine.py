@@ -113,6 +113,8 @@ _SIGNATURES = {
     "qk_mc_last_timing": (_I, [_PD, _PD]),
     "qk_mc_validation_stream": (_I, [_U, ct.c_int64, _PD]),
     "qk_device_info": (_I, [ct.c_char_p, _I, _PI]),
+    "qk_mc_philox_probe": (_I, [_U, ct.c_uint32, ct.c_uint32, ct.c_uint32, _I,
+                                ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_float)]),
 }
 
 
@@ -195,6 +197,15 @@ class QuantKernel:
         self._check(self._lib.qk_mc_validation_stream(int(seed), int(count), out.ctypes.data_as(_PD)),
                     "qk_mc_validation_stream")
         return out
+
+    def philox_probe(self, seed: int, c0: int, c1: int, c3: int, count: int):
+        words = np.empty(4 * count, dtype=np.uint32)
+        normals = np.empty(4 * count, dtype=np.float32)
+        self._check(self._lib.qk_mc_philox_probe(int(seed), int(c0), int(c1), int(c3), int(count),
+                                                 words.ctypes.data_as(ct.POINTER(ct.c_uint32)),
+                                                 normals.ctypes.data_as(ct.POINTER(ct.c_float))),
+                    "qk_mc_philox_probe")
+        return words.reshape(count, 4), normals.reshape(count, 4)
 
     # ---- helpers (engine.py:99-124, 163-194) -----------------------------
     @staticmethod

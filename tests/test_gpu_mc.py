@@ -281,3 +281,19 @@ def test_lsmc_philox_matches_fp64_oracle_and_crr(qk, orc):
         print(i, got[i], w, se[i], crr)
         assert abs(got[i] - w) < 0.25 * se[i] + 1e-4 * a[0]
         assert abs(got[i] - crr) < 4 * se[i] + 0.02
+
+
+def test_device_philox_bitwise_and_normals(qk, orc):
+    """The device generator equals the oracle's Philox4x32-10 word for word,
+    and its FP32 Box-Muller normals equal the FP64 ones to FP32 accuracy."""
+    import ctypes as ct
+    for seed, c0, c1, c3 in [(0, 0, 0, 0), (0xFFFFFFFFFFFFFFFF, 7, 3, 1), (0x299F31D0A4093822, 1000, 77, 0)]:
+        words, normals = qk.philox_probe(seed, c0, c1, c3, 512)
+        for i in (0, 1, 255, 511):
+            want = orc.philox([(c0 + i) & 0xFFFFFFFF, c1, 0, c3], [seed & 0xFFFFFFFF, seed >> 32])
+            assert list(words[i]) == want
+            for j in (0, 2):
+                a, b = ct.c_double(), ct.c_double()
+                orc.lib.orc_box_muller(int(words[i][j]), int(words[i][j + 1]), ct.byref(a), ct.byref(b))
+                assert abs(normals[i][j] - a.value) < 2e-6 * max(1.0, abs(a.value))
+                assert abs(normals[i][j + 1] - b.value) < 2e-6 * max(1.0, abs(b.value))

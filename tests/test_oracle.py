@@ -121,10 +121,14 @@ def test_box_muller_distribution(orc):
 def test_box_muller_exact_mapping(orc):
     import ctypes as ct
     z0, z1 = ct.c_double(), ct.c_double()
-    # top 24 bits: u1 = (0x800000 + .5)/2^24 (~1/2), u2 = 0.25 -> angle pi/2
-    orc.lib.orc_box_muller(0x80000000, 0x40000000, ct.byref(z0), ct.byref(z1))
-    rad = math.sqrt(-2 * math.log((0x800000 + 0.5) / 2 ** 24))
+    # a = 2^31: u1 = fmaf(2^31, 2^-32, 2^-33) rounds to 0.5 in FP32;
+    # b low 23 bits = 2^21: f = 1/4, angle pi/2
+    orc.lib.orc_box_muller(0x80000000, 0x40200000, ct.byref(z0), ct.byref(z1))
+    rad = math.sqrt(2 * math.log(2))
     assert z0.value == pytest.approx(0.0, abs=1e-12) and z1.value == pytest.approx(rad, rel=1e-14)
+    # smallest u1: a = 0 -> 2^-33, the largest radius
+    orc.lib.orc_box_muller(0, 0, ct.byref(z0), ct.byref(z1))
+    assert z0.value == pytest.approx(math.sqrt(2 * 33 * math.log(2)), rel=1e-14) and z1.value == 0.0
 
 
 def test_bsm_known_values_and_parity(orc):
