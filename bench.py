@@ -1,20 +1,30 @@
-"""Benchmark of the Heston Monte Carlo batch path on one MI355X.
+"""Benchmark of the Heston pricing path on one MI355X.
 
-Workload: 16384 Heston full-truncation Euler European calls, 65536 paths and
-256 steps each (seeded synthetic parameters, see quantkernel_amd.workloads).
-A step is one pass of the whole batch through the library's device-resident
-entry point (inputs already in HBM): prepare -> tile scan -> persistent
-Philox/FP32 path kernel -> finalize.  The CPU baseline times the reference's
-own batch entry point (a build of its sources, -O3 -fopenmp) on the first
-options of the same batch.
+Default workload (the headline metric): 1,048,576 seeded Heston European
+call queries, 95% drawn from the surrogate's training box and 5% from the box
+widened by 10% on every side.  One step prices the whole batch through
+qk_sur_heston_price_batch_device: network for in-box queries, plain MC
+(65536 paths x 256 steps, Philox, FP32 paths, FP64 sums) for queries outside
+the box and for spot-check disagreements (every 128th answered query is
+re-priced with 8192 paths).  Inputs are resident in HBM before timing.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Alongside, in the same run:
+  * plain GPU MC at the same accuracy: the same MC kernel at 65536 paths
+    (median SE ~6.7e-4 of spot, below the 1e-3 target) on a 4096-query slice;
+  * accuracy of both against MC at 4,194,304 paths on 1024 in-box queries;
+  * the reference's CPU batch entry point (a build of its sources, -O3
+    -fopenmp) on the first queries, same paths and steps.
+
+--workload c1 runs the plain MC configuration instead: 16384 Heston calls x
+65536 paths x 256 steps.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c1]
 """
 import argparse
 import ctypes as ct
 import json
+import math
 import os
-import subprocess
 import sys
 import time
 from pathlib import Path
@@ -24,148 +34,253 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "quant-kernel_amd"))
 
-PEAK_FP32_TFLOPS = 157.3       # MI355X published FP32 vector peak
-FLOPS_PER_PATH_STEP = 37       # Heston step incl. two Box-Muller normals (algorithmic count)
+PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 vector peak
+PEAK_BF16_TFLOPS = 2500.0     # MI355X published dense BF16 MFMA peak
+FLOPS_PER_PATH_STEP = 37      # Heston step incl. two Box-Muller normals (algorithmic count)
+MC_PATHS = 65536
+SPOT_EVERY, SPOT_PATHS = 128, 8192
+STEPS = 256
+METRIC = "Heston prices/sec at <=1e-3 rel err: surrogate+fallback vs plain MC, 1 GPU"
+KEYS = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_info():
-    model = ""
+def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return model, os.cpu_count() or 1
+    return "unknown"
 
 
 def cpu_baseline(d, budget_s=12.0):
     """Times the reference build (or, without it, the oracle port) on the
-    first k options of the batch, full paths and steps each."""
+    first k queries, full paths and steps each."""
     ref = ROOT / "oracle" / "_ref" / "libqkref.so"
-    kind = "reference"
     if ref.exists():
-        lib = ct.CDLL(str(ref))
-        fn = lib.qk_mcm_heston_monte_carlo_price_batch
+        kind, fn = "reference", ct.CDLL(str(ref)).qk_mcm_heston_monte_carlo_price_batch
     else:
-        kind = "port"
-        lib = ct.CDLL(str(ROOT / "oracle" / "liboracle.so"))
-        fn = lib.orc_heston_batch
+        kind, fn = "port", ct.CDLL(str(ROOT / "oracle" / "liboracle.so")).orc_heston_batch
     PD, PI, PU = ct.POINTER(ct.c_double), ct.POINTER(ct.c_int32), ct.POINTER(ct.c_uint64)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
 
     def run(k):
-        arrs = [np.ascontiguousarray(d[key][:k]) for key in keys]
-        ot = np.ascontiguousarray(d["option_type"][:k])
-        p = np.ascontiguousarray(d["paths"][:k])
-        st = np.ascontiguousarray(d["steps"][:k])
+        arrs = [np.ascontiguousarray(d[key][:k]) for key in KEYS]
+        ints = [np.ascontiguousarray(d[key][:k]) for key in ("option_type", "paths", "steps")]
         sd = np.ascontiguousarray(d["seed"][:k])
         out = np.empty(k)
-        args = [a.ctypes.data_as(PD) for a in arrs] + [ot.ctypes.data_as(PI), p.ctypes.data_as(PI),
-                                                       st.ctypes.data_as(PI), sd.ctypes.data_as(PU),
-                                                       ct.c_int32(k), out.ctypes.data_as(PD)]
         t0 = time.perf_counter()
-        fn(*args)
-        return time.perf_counter() - t0, out
+        fn(*[a.ctypes.data_as(PD) for a in arrs], *[a.ctypes.data_as(PI) for a in ints], sd.ctypes.data_as(PU),
+           ct.c_int32(k), out.ctypes.data_as(PD))
+        return time.perf_counter() - t0
 
     k = max(1, threads)
-    dt, _ = run(k)
+    dt = run(k)
     log(f"cpu baseline probe: {k} options in {dt:.2f}s")
     if dt < budget_s / 3:
-        k2 = int(k * budget_s / max(dt, 1e-3))
-        k2 = max(k, (k2 // threads) * threads)
-        dt, _ = run(k2)
-        k = k2
+        k = max(k, int(k * budget_s / max(dt, 1e-3)) // threads * threads)
+        dt = run(k)
         log(f"cpu baseline: {k} options in {dt:.2f}s")
     return {"value": k / dt, "unit": "prices/s", "cores": threads, "kind": kind,
-            "sample": f"first {k} options of the batch, 65536 paths x 256 steps each, "
-                      f"OpenMP over options ({threads} threads, {cpu_info()[0]})"}
+            "sample": f"first {k} queries of the batch priced by the reference's Heston MC batch entry point "
+                      f"({MC_PATHS} paths x {STEPS} steps each, OpenMP over options, {threads} threads, "
+                      f"{cpu_model()}); the reference has no surrogate"}
+
+
+def to_device(d, dev):
+    import torch
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in d.items()}
+
+
+def plain_mc_rate(qk, qka, d, n, dev, reps=2):
+    """Plain GPU MC (device-resident) on the first n queries: prices/s and
+    the simulate-kernel time."""
+    import torch
+    sl = {k: v[:n] for k, v in d.items()}
+    sl["model"] = np.full(n, qka.QK_MODEL_HESTON_EULER, np.int32)
+    sl["vol"] = np.zeros(n)
+    t = to_device(sl, dev)
+    ptrs = {k: v.data_ptr() for k, v in t.items()}
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    se = torch.empty(n, dtype=torch.float64, device=dev)
+    qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sims = []
+    for _ in range(reps):
+        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+        sims.append(qk.last_timing()[1])
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    return n / wall, float(np.mean(sims)), out.cpu().numpy(), se.cpu().numpy()
+
+
+def bench_c1(args, qk, qka, dev, name, cus):
+    import torch
+    from quantkernel_amd import workloads as wl
+    n = args.options or 16384
+    d = wl.config1(n)
+    d["model"] = np.full(n, qka.QK_MODEL_HESTON_EULER, np.int32)
+    d["vol"] = np.zeros(n)
+    t = to_device(d, dev)
+    ptrs = {k: v.data_ptr() for k, v in t.items()}
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    se = torch.empty(n, dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+    torch.cuda.synchronize()
+    sims = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+        sims.append(qk.last_timing()[1])
+        log(f"step done: sim kernel {sims[-1]:.2f} ms")
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    sim_ms = float(np.mean(sims))
+    flops = float(n) * MC_PATHS * STEPS * FLOPS_PER_PATH_STEP
+    ach = flops / (sim_ms * 1e-3) / 1e12
+    res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
+               config={"workload": f"plain GPU MC: {n} Heston full-truncation Euler calls x {MC_PATHS} paths x "
+                                   f"{STEPS} steps (Philox, FP32 paths, FP64 sums)", "options": n,
+                       "paths": MC_PATHS, "time_steps": STEPS, "device": name, "cus": cus,
+                       "median_stderr_over_spot": float(np.median(se.cpu().numpy() / d["spot"]))},
+               roofline={"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                         "kernel": "qk::k_sim_stepwise<float, PHILOX>", "kernel_ms": sim_ms,
+                         "flops_per_launch": flops})
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(d)
+    return res
+
+
+def bench_c4(args, qk, qka, dev, name, cus):
+    import torch
+    from quantkernel_amd import surrogate as sur
+    weights = Path(args.weights)
+    qk.sur_load(weights)
+    qk.sur_configure(spot_every=SPOT_EVERY, spot_paths=SPOT_PATHS, spot_k_se=3.0)
+    n = args.options or 1 << 20
+    n_out = n // 20
+    d = sur.sample_queries(n - n_out, sur.MASTER_QUERY)
+    e = sur.sample_queries(n_out, sur.MASTER_QUERY + 1, expand=0.1)
+    d = {k: np.concatenate([d[k], e[k]]) for k in d}
+    d["paths"] = np.full(n, MC_PATHS, np.int32)
+    t = to_device(d, dev)
+    ptrs = {k: v.data_ptr() for k, v in t.items()}
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    src = torch.empty(n, dtype=torch.int32, device=dev)
+
+    def step():
+        qk.sur_heston_price_batch_device(n, ptrs, out.data_ptr(), src.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    net_ms, mc_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        a, b = qk.sur_last_timing()
+        net_ms.append(a)
+        mc_ms.append(b)
+        log(f"step done: network {a:.2f} ms, MC {b:.2f} ms")
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    source = src.cpu().numpy()
+    n_net = int(np.sum(source == 0))
+    n_box = int(np.sum(source == 1))
+    n_chk = int(np.sum(source == 2))
+    n_spot = int(np.sum((np.arange(n) % SPOT_EVERY == 0) & ((source == 0) | (source == 2))))
+    mc_flops = (float(n_box + n_chk) * MC_PATHS + float(n_spot) * SPOT_PATHS) * STEPS * FLOPS_PER_PATH_STEP
+    mc_ms_avg = float(np.mean(mc_ms))
+    ach = mc_flops / (mc_ms_avg * 1e-3) / 1e12
+    net_flops = 2.0 * n * (32 * 256 + 3 * 256 * 256 + 256)
+    net_ms_avg = float(np.mean(net_ms))
+
+    # plain GPU MC at the same accuracy, and accuracy of both, same run
+    plain_rate, plain_sim_ms, _, _ = plain_mc_rate(qk, qka, d, 4096, dev)
+    log(f"plain MC: {plain_rate:.0f} prices/s")
+    na = args.accuracy_queries
+    acc = {}
+    if na > 0:
+        a = {k: v[:na] for k, v in d.items()}
+        a["option_type"] = (np.arange(na) % 2).astype(np.int32)
+        ref, ref_se = sur.mc_price(qk, a, 1 << 22, chunk=2048)
+        mc, mc_se = qk.heston_monte_carlo_price_batch(*[a[k] for k in KEYS], a["option_type"],
+                                                      np.full(na, MC_PATHS, np.int32), a["steps"],
+                                                      a["seed"] + np.uint64(7), return_stderr=True)
+        qk.sur_configure(spot_every=0)
+        sp, ss = qk.sur_heston_price_batch(*[a[k] for k in KEYS], a["option_type"], a["paths"], a["steps"],
+                                           a["seed"])
+        qk.sur_configure(spot_every=SPOT_EVERY, spot_paths=SPOT_PATHS, spot_k_se=3.0)
+        es = (sp - ref) / a["spot"]
+        em = (mc - ref) / a["spot"]
+        acc = {"queries": na, "reference_paths": 1 << 22,
+               "reference_median_se_over_spot": float(np.median(ref_se / a["spot"])),
+               "surrogate_rmse_over_spot": float(np.sqrt(np.mean(es ** 2))),
+               "surrogate_p99_abs_err_over_spot": float(np.quantile(np.abs(es), 0.99)),
+               "plain_mc_rmse_over_spot": float(np.sqrt(np.mean(em ** 2))),
+               "plain_mc_p99_abs_err_over_spot": float(np.quantile(np.abs(em), 0.99)),
+               "answered_by_network": int(np.sum(ss == 0))}
+        log(json.dumps(acc))
+    value = n / (ms * 1e-3)
+    res = dict(value=value, ms_per_step=ms, dtype="fp32",
+               config={"workload": f"Heston surrogate + MC fallback: {n} call queries (95% in the training box, "
+                                   f"5% from the box widened 10% per side); fallback and spot checks by plain MC "
+                                   f"({MC_PATHS} / {SPOT_PATHS} paths x {STEPS} steps, Philox, FP32 paths)",
+                       "queries": n, "answered_by_network": n_net, "mc_fallback_out_of_box": n_box,
+                       "mc_fallback_spot_check": n_chk, "spot_checked": n_spot, "device": name, "cus": cus,
+                       "weights": weights.name, "network_ms": net_ms_avg, "mc_ms": mc_ms_avg,
+                       "network_mfma_tflops": net_flops / (net_ms_avg * 1e-3) / 1e12,
+                       "network_mfma_frac_bf16_peak": net_flops / (net_ms_avg * 1e-3) / 1e12 / PEAK_BF16_TFLOPS,
+                       "plain_gpu_mc_prices_per_s": plain_rate, "gain_vs_plain_gpu_mc": value / plain_rate,
+                       "accuracy": acc},
+               roofline={"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                         "kernel": "qk::k_sim_stepwise<float, PHILOX> (MC fallback + spot checks)",
+                         "kernel_ms": mc_ms_avg, "flops_per_launch": mc_flops})
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(d)
+    return res
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--options", type=int, default=16384)
+    ap.add_argument("--workload", choices=["c4", "c1"], default=None)
+    ap.add_argument("--options", type=int, default=0)
+    ap.add_argument("--weights", default=str(ROOT / "quant-kernel_amd" / "quantkernel_amd" / "heston_surrogate.qkw"))
+    ap.add_argument("--accuracy-queries", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(options shard independently; see the design notes).", file=sys.stderr)
+              "(options shard independently across GPUs; see the design notes).", file=sys.stderr)
         sys.exit(2)
-
     import torch
 
     import quantkernel_amd as qka
-    from quantkernel_amd import workloads as wl
 
     qk = qka.QuantKernel()
     name, cus = qk.device_info()
-    n = args.options
-    d = wl.config1(n)
-    d["model"] = np.full(n, qka.QK_MODEL_HESTON_EULER, dtype=np.int32)
-    d["vol"] = np.zeros(n)
     dev = torch.device("cuda:0")
-    tens = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in d.items()}
-    ptrs = {k: t.data_ptr() for k, t in tens.items()}
-    out = torch.empty(n, dtype=torch.float64, device=dev)
-    se = torch.empty(n, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-
-    def step():
-        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    sims = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        sims.append(qk.last_timing()[1])
-        log(f"step done: sim kernel {sims[-1]:.2f} ms")
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = wall * 1e3 / args.steps
-    price = out.cpu().numpy()
-    stderr = se.cpu().numpy()
-    assert np.all(np.isfinite(price)), "non-finite prices"
-
-    sim_ms = float(np.mean(sims))
-    flops = float(n) * 65536 * 256 * FLOPS_PER_PATH_STEP
-    achieved = flops / (sim_ms * 1e-3) / 1e12
-    result = {
-        "metric": "Heston prices/sec at <=1e-3 rel err: surrogate+fallback vs plain MC, 1 GPU",
-        "value": n / (ms_step * 1e-3),
-        "unit": "prices/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic",
-        "config": {"workload": f"plain GPU MC, Heston full-truncation Euler calls, {n} options x 65536 "
-                               f"paths x 256 steps (Philox, FP32 paths, FP64 sums)",
-                   "options": n, "paths": 65536, "time_steps": 256, "device": name, "cus": cus,
-                   "median_stderr_over_spot": float(np.median(stderr / d["spot"]))},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_sim_stepwise<float,PHILOX>", "kernel_ms": sim_ms,
-                     "flops_per_launch": flops},
-    }
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(d)
-    print(json.dumps(result), flush=True)
+    workload = args.workload or ("c4" if Path(args.weights).exists() else "c1")
+    res = (bench_c4 if workload == "c4" else bench_c1)(args, qk, qka, dev, name, cus)
+    line = {"metric": METRIC, "value": res["value"], "unit": "prices/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": res["dtype"], "data": "synthetic", "config": res["config"],
+            "roofline": res["roofline"]}
+    if "cpu_baseline" in res:
+        line["cpu_baseline"] = res["cpu_baseline"]
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

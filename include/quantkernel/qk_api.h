@@ -214,6 +214,55 @@ QK_EXPORT int32_t qk_mc_philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, ui
  * if there is none. */
 QK_EXPORT int32_t qk_device_info(char* name_buf, int32_t buf_len, int32_t* compute_units);
 
+/* ---- Heston surrogate (ABI 2.13; no reference counterpart) -------------
+ * An MLP (8 -> 4 x 256 GELU -> 1; BF16 MFMA, FP32 accumulate) maps
+ * (log K/S, T, r, kappa, theta, sigma, rho, v0) to the price/S of a European
+ * call under the Heston full-truncation Euler scheme with 256 steps.  Features
+ * are passed raw (n x 8, row-major); the library maps its training box
+ * log K/S in [ln .7, ln 1.3], T in [.05, 2], r in [0, .08], kappa in [.5, 5],
+ * theta in [.01, .16], sigma in [.1, 1], rho in [-.9, -.1], v0 in [.01, .16]
+ * to [-1, 1].  The model is a closed-form control (the Black-Scholes call
+ * per unit spot at the expected average variance
+ * theta + (v0 - theta)(1 - e^{-kappa T})/(kappa T)) plus the network:
+ * qk_sur_train takes full prices/S and fits the network to the difference,
+ * qk_sur_predict returns control + network, and qk_sur_gradients returns the
+ * network's own MSE gradients against the given network targets.  Weights
+ * live on the device; one model per process. */
+QK_EXPORT int32_t qk_sur_param_count(void);
+QK_EXPORT int32_t qk_sur_init(uint64_t seed);
+QK_EXPORT int32_t qk_sur_train(const double* features, const double* targets, int32_t n,
+                               int32_t epochs, int32_t batch, double lr, double lr_final,
+                               uint64_t seed, double* out_epoch_loss);
+QK_EXPORT int32_t qk_sur_predict(const double* features, int32_t n, double* out);
+QK_EXPORT int32_t qk_sur_gradients(const double* features, const double* targets, int32_t n,
+                                   float* out_grads, double* out_loss);
+QK_EXPORT int32_t qk_sur_get_params(float* out);
+QK_EXPORT int32_t qk_sur_set_params(const float* params);
+QK_EXPORT int32_t qk_sur_save(const char* path);
+QK_EXPORT int32_t qk_sur_load(const char* path);
+/* Spot check: every spot_every-th query answered by the network is also
+ * priced by plain MC with spot_paths paths; a difference above
+ * spot_k_se * SE sends it to the full MC fallback.  spot_every = 0 disables. */
+QK_EXPORT int32_t qk_sur_configure(int32_t spot_every, int32_t spot_paths, double spot_k_se);
+/* Heston prices with the same arguments as qk_mcm_heston_monte_carlo_price_batch:
+ * queries inside the box (q == 0, steps == 256) are answered by the network
+ * (puts by put-call parity), the others by plain MC with their own paths,
+ * steps and seed; out_source[i] is 0 network, 1 MC (outside the box),
+ * 2 MC (spot check failed), 3 invalid / t <= eps (NaN / intrinsic). */
+QK_EXPORT int32_t qk_sur_heston_price_batch(const double* spot, const double* strike,
+        const double* t, const double* r, const double* q, const double* v0,
+        const double* kappa, const double* theta, const double* sigma, const double* rho,
+        const int32_t* option_type, const int32_t* paths, const int32_t* steps,
+        const uint64_t* seed, int32_t n, double* out_prices, int32_t* out_source);
+/* Same with device-resident inputs and outputs. */
+QK_EXPORT int32_t qk_sur_heston_price_batch_device(const double* spot, const double* strike,
+        const double* t, const double* r, const double* q, const double* v0,
+        const double* kappa, const double* theta, const double* sigma, const double* rho,
+        const int32_t* option_type, const int32_t* paths, const int32_t* steps,
+        const uint64_t* seed, int32_t n, double* out_prices, int32_t* out_source);
+/* Device time of the last surrogate call: network phase and MC phase (ms). */
+QK_EXPORT int32_t qk_sur_last_timing(double* ms_network, double* ms_monte_carlo);
+
 #ifdef __cplusplus
 }
 #endif

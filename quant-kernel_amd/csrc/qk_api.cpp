@@ -15,6 +15,8 @@
 
 #include "qk_engine.hpp"
 #include "qk_lsmc.hpp"
+#include "qk_mlp.hpp"
+#include "qk_surrogate.hpp"
 #include "quantkernel/qk_api.h"
 
 namespace {
@@ -470,6 +472,140 @@ double qk_mcm_longstaff_schwartz_price(double spot, double strike, double t, dou
                                           nullptr, &err);
     if (rc != QK_OK) set_msg(err);
     return scalar_of(rc, out);
+}
+
+}  // extern "C"
+
+// ---- Heston surrogate (ABI 2.13) ------------------------------------------
+extern "C" {
+
+#define QK_SUR_RC(expr)                    \
+    do {                                   \
+        std::string err_;                  \
+        const int32_t rc_ = (expr);        \
+        if (rc_ != QK_OK) set_msg(err_);   \
+        return rc_;                        \
+    } while (0)
+
+int32_t qk_sur_param_count(void) { return qk::mlp::kParams; }
+
+int32_t qk_sur_init(uint64_t seed) { QK_SUR_RC(qk::sur_init(seed, &err_)); }
+
+int32_t qk_sur_train(const double* features, const double* targets, int32_t n, int32_t epochs,
+                     int32_t batch, double lr, double lr_final, uint64_t seed,
+                     double* out_epoch_loss) {
+    if (!check_ptrs({QK_P(features), QK_P(targets)})) return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    QK_SUR_RC(qk::sur_train(features, targets, n, epochs, batch, lr, lr_final, seed,
+                            out_epoch_loss, &err_));
+}
+
+int32_t qk_sur_predict(const double* features, int32_t n, double* out) {
+    if (!check_ptrs({QK_P(features), QK_P(out)})) return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    QK_SUR_RC(qk::sur_predict(features, n, out, &err_));
+}
+
+int32_t qk_sur_gradients(const double* features, const double* targets, int32_t n,
+                         float* out_grads, double* out_loss) {
+    if (!check_ptrs({QK_P(features), QK_P(targets), QK_P(out_grads), QK_P(out_loss)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    QK_SUR_RC(qk::sur_gradients(features, targets, n, out_grads, out_loss, &err_));
+}
+
+int32_t qk_sur_get_params(float* out) {
+    if (!check_ptrs({QK_P(out)})) return QK_ERR_NULL_PTR;
+    QK_SUR_RC(qk::sur_get_params(out, &err_));
+}
+
+int32_t qk_sur_set_params(const float* params) {
+    if (!check_ptrs({QK_P(params)})) return QK_ERR_NULL_PTR;
+    QK_SUR_RC(qk::sur_set_params(params, &err_));
+}
+
+int32_t qk_sur_save(const char* path) {
+    if (!check_ptrs({QK_P(path)})) return QK_ERR_NULL_PTR;
+    QK_SUR_RC(qk::sur_save(path, &err_));
+}
+
+int32_t qk_sur_load(const char* path) {
+    if (!check_ptrs({QK_P(path)})) return QK_ERR_NULL_PTR;
+    QK_SUR_RC(qk::sur_load(path, &err_));
+}
+
+int32_t qk_sur_configure(int32_t spot_every, int32_t spot_paths, double spot_k_se) {
+    if (spot_every < 0 || spot_paths < 2 || !(spot_k_se > 0.0)) {
+        set_msg("surrogate spot check needs spot_every >= 0, spot_paths >= 2, spot_k_se > 0");
+        return QK_ERR_INVALID_INPUT;
+    }
+    qk::sur_configure(spot_every, spot_paths, spot_k_se);
+    return QK_OK;
+}
+
+static int32_t sur_price_common(bool device, const double* spot, const double* strike,
+                                const double* t, const double* r, const double* q,
+                                const double* v0, const double* kappa, const double* theta,
+                                const double* sigma, const double* rho,
+                                const int32_t* option_type, const int32_t* paths,
+                                const int32_t* steps, const uint64_t* seed, int32_t n,
+                                double* out_prices, int32_t* out_source) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(r), QK_P(q), QK_P(v0), QK_P(kappa),
+                     QK_P(theta), QK_P(sigma), QK_P(rho), QK_P(option_type), QK_P(paths),
+                     QK_P(steps), QK_P(seed), QK_P(out_prices), QK_P(out_source)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    if (qk::get_rng_mode() != QK_RNG_PHILOX) {
+        set_msg("surrogate pricing uses the Philox MC fallback: set QK_RNG_PHILOX");
+        return QK_ERR_INVALID_INPUT;
+    }
+    qk::BatchIn bi{};
+    bi.spot = spot;
+    bi.strike = strike;
+    bi.t = t;
+    bi.r = r;
+    bi.q = q;
+    bi.v0 = v0;
+    bi.kappa = kappa;
+    bi.theta = theta;
+    bi.sigma = sigma;
+    bi.rho = rho;
+    bi.ot = option_type;
+    bi.paths = paths;
+    bi.steps = steps;
+    bi.seed = seed;
+    QK_SUR_RC(qk::sur_heston_price(bi, n, device, out_prices, out_source, &err_));
+}
+
+int32_t qk_sur_heston_price_batch(const double* spot, const double* strike, const double* t,
+                                  const double* r, const double* q, const double* v0,
+                                  const double* kappa, const double* theta, const double* sigma,
+                                  const double* rho, const int32_t* option_type,
+                                  const int32_t* paths, const int32_t* steps,
+                                  const uint64_t* seed, int32_t n, double* out_prices,
+                                  int32_t* out_source) {
+    return sur_price_common(false, spot, strike, t, r, q, v0, kappa, theta, sigma, rho,
+                            option_type, paths, steps, seed, n, out_prices, out_source);
+}
+
+int32_t qk_sur_heston_price_batch_device(const double* spot, const double* strike,
+                                         const double* t, const double* r, const double* q,
+                                         const double* v0, const double* kappa,
+                                         const double* theta, const double* sigma,
+                                         const double* rho, const int32_t* option_type,
+                                         const int32_t* paths, const int32_t* steps,
+                                         const uint64_t* seed, int32_t n, double* out_prices,
+                                         int32_t* out_source) {
+    return sur_price_common(true, spot, strike, t, r, q, v0, kappa, theta, sigma, rho,
+                            option_type, paths, steps, seed, n, out_prices, out_source);
+}
+
+int32_t qk_sur_last_timing(double* ms_network, double* ms_monte_carlo) {
+    if (!qk::sur_last_timing(ms_network, ms_monte_carlo)) {
+        set_msg("no surrogate pricing call has completed yet");
+        return QK_ERR_RUNTIME_INIT;
+    }
+    return QK_OK;
 }
 
 }  // extern "C"

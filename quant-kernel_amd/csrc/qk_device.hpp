@@ -1,0 +1,103 @@
+// qk_device.hpp - the process-wide device context shared by the MC engine
+// and the surrogate: HIP device, one default-priority stream, timing events,
+// grow-only device / pinned buffers and the lock that serialises device work.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <mutex>
+#include <string>
+
+#include "qk_kernels.hpp"
+
+namespace qk {
+
+#define QK_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            if (err) *err = std::string("HIP error: ") + hipGetErrorString(e_) + " (" + \
+                            #call + ")";                                               \
+            return QK_ERR_RUNTIME_INIT;                                                \
+        }                                                                              \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 4096);
+        want = (want + 4095) & ~size_t(4095);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+};
+
+// Grow-only pinned host buffer.
+struct PinBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipHostFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 4096);
+        want = (want + 4095) & ~size_t(4095);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+};
+
+inline int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return std::atoi(v);
+}
+
+struct Device {
+    int dev = 0;
+    int cus = 0;
+    std::string name;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    hipEvent_t ev_sur[3] = {};  // surrogate pricing phases
+    std::mutex mu;
+
+    DevBuf in, recs, ntiles, tile_start, counters, partials, out, stream_buf, stream_off;
+    DevBuf lsmc_path, lsmc_cf, lsmc_recs;
+    PinBuf pin_in, pin_out;
+
+    double ms_total = -1.0, ms_sim = -1.0;
+
+    SimShape shape_step{8, 0};
+    SimShape shape_term{8, 0};
+};
+
+// Returns the context (creating it on first use) or null with *err set.
+Device* get_device(std::string* err);
+
+}  // namespace qk

@@ -5,6 +5,7 @@
 // and pinned host buffers.  Device work is serialised by a mutex; the C ABI
 // may be entered from several host threads.
 #include "qk_engine.hpp"
+#include "qk_device.hpp"
 #include "qk_lsmc.hpp"
 
 #include <hip/hip_runtime.h>
@@ -24,88 +25,6 @@ namespace qk {
 namespace {
 
 std::atomic<int32_t> g_rng_mode{QK_RNG_PHILOX};
-
-#define QK_HIP(call)                                                                   \
-    do {                                                                               \
-        hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess) {                                                        \
-            if (err) *err = std::string("HIP error: ") + hipGetErrorString(e_) + " (" + \
-                            #call + ")";                                               \
-            return QK_ERR_RUNTIME_INIT;                                                \
-        }                                                                              \
-    } while (0)
-
-// Grow-only device buffer.
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            if (e != hipSuccess) return e;
-            p = nullptr;
-            cap = 0;
-        }
-        size_t want = std::max<size_t>(bytes, 4096);
-        want = (want + 4095) & ~size_t(4095);
-        hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return e;
-        }
-        cap = want;
-        return hipSuccess;
-    }
-};
-
-// Grow-only pinned host buffer.
-struct PinBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap) return hipSuccess;
-        if (p) {
-            hipError_t e = hipHostFree(p);
-            if (e != hipSuccess) return e;
-            p = nullptr;
-            cap = 0;
-        }
-        size_t want = std::max<size_t>(bytes, 4096);
-        want = (want + 4095) & ~size_t(4095);
-        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return e;
-        }
-        cap = want;
-        return hipSuccess;
-    }
-};
-
-int env_int(const char* name, int dflt) {
-    const char* v = std::getenv(name);
-    if (!v || !*v) return dflt;
-    return std::atoi(v);
-}
-
-struct Device {
-    int dev = 0;
-    int cus = 0;
-    std::string name;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {};
-    std::mutex mu;
-
-    DevBuf in, recs, ntiles, tile_start, counters, partials, out, stream_buf, stream_off;
-    DevBuf lsmc_path, lsmc_cf, lsmc_recs;
-    PinBuf pin_in, pin_out;
-
-    double ms_total = -1.0, ms_sim = -1.0;
-
-    SimShape shape_step{8, 0};
-    SimShape shape_term{8, 0};
-};
 
 Device* g_dev = nullptr;
 std::string g_init_err;
@@ -146,6 +65,7 @@ void init_device() {
     d->name = prop.name;
     e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&d->ev[i]);
+    for (int i = 0; e == hipSuccess && i < 3; ++i) e = hipEventCreate(&d->ev_sur[i]);
     if (e != hipSuccess) {
         g_init_err = std::string("stream/event creation: ") + hipGetErrorString(e);
         delete d;
@@ -159,11 +79,15 @@ void init_device() {
     g_dev = d;
 }
 
+}  // namespace
+
 Device* get_device(std::string* err) {
     std::call_once(g_once, init_device);
     if (!g_dev && err) *err = g_init_err;
     return g_dev;
 }
+
+namespace {
 
 int32_t method_of(const HostBatch& hb, int32_t i) {
     if (!hb.model_ids) return hb.uniform_method;
@@ -308,25 +232,10 @@ static void fill_streams(const HostBatch& hb, int32_t n, const std::vector<Optio
     for (auto& th : pool) th.join();
 }
 
-int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* out_se,
-                     std::string* err) {
-    Device* d = get_device(err);
-    if (!d) return QK_ERR_RUNTIME_INIT;
-    std::lock_guard<std::mutex> lock(d->mu);
-    QK_HIP(hipSetDevice(d->dev));
-
-    const int32_t mode = get_rng_mode();
-    // family of the launch: a batch is all-terminal or all-stepwise
-    int32_t family = FAM_STEPWISE;
-    if (!hb.model_ids && is_terminal_method(hb.uniform_method)) family = FAM_TERMINAL;
-
-    const int64_t tiles = tile_bound(hb, n, d);
-    if (tiles > 0x7fffffffLL) {
-        if (err) *err = "batch too large: more than 2^31-1 path tiles";
-        return QK_ERR_INVALID_INPUT;
-    }
-
-    // ---- pack inputs (SoA, stride ns) into pinned memory, one H2D copy ----
+// Packs a host SoA batch (stride align64(n)) into pinned memory and queues
+// one H2D copy into the device staging buffer; fills *bi and *n_dev.
+int32_t stage_host_batch(Device* d, const HostBatch& hb, int32_t n, BatchIn* bi_out,
+                         const int32_t** n_dev_out, std::string* err) {
     const size_t ns = align64(size_t(n));
     const double* dsrc[11] = {hb.spot, hb.strike, hb.t, hb.vol, hb.r, hb.q,
                               hb.v0, hb.kappa, hb.theta, hb.sigma, hb.rho};
@@ -367,6 +276,35 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
     off += 64;
     bi.uniform_method = hb.uniform_method;
     bi.method_is_model = hb.model_ids ? 1 : 0;
+    QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, off, hipMemcpyHostToDevice, d->stream));
+    *bi_out = bi;
+    *n_dev_out = n_dev;
+    return QK_OK;
+}
+
+int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* out_se,
+                     std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+
+    const int32_t mode = get_rng_mode();
+    // family of the launch: a batch is all-terminal or all-stepwise
+    int32_t family = FAM_STEPWISE;
+    if (!hb.model_ids && is_terminal_method(hb.uniform_method)) family = FAM_TERMINAL;
+
+    const int64_t tiles = tile_bound(hb, n, d);
+    if (tiles > 0x7fffffffLL) {
+        if (err) *err = "batch too large: more than 2^31-1 path tiles";
+        return QK_ERR_INVALID_INPUT;
+    }
+
+    const size_t ns = align64(size_t(n));
+    BatchIn bi{};
+    const int32_t* n_dev = nullptr;
+    int32_t prc = stage_host_batch(d, hb, n, &bi, &n_dev, err);
+    if (prc != QK_OK) return prc;
 
     // ---- device work buffers ----
     QK_HIP(d->recs.ensure(ns * sizeof(OptionRec)));
@@ -437,7 +375,6 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
         d_stream = d->stream_buf.p;
     }
 
-    QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, off, hipMemcpyHostToDevice, d->stream));
     int32_t rc = launch_pipeline(d, bi, n_dev, d_off, family, sim_mode, d_stream, dprice, dse, err);
     if (rc != QK_OK) return rc;
     // outputs + the scan's overflow flag
@@ -457,27 +394,8 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
     return QK_OK;
 }
 
-int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
-                            double* out_se_dev, std::string* err) {
-    Device* d = get_device(err);
-    if (!d) return QK_ERR_RUNTIME_INIT;
-    if (get_rng_mode() != QK_RNG_PHILOX) {
-        if (err) *err = "device-resident batches support QK_RNG_PHILOX only";
-        return QK_ERR_INVALID_INPUT;
-    }
-    std::lock_guard<std::mutex> lock(d->mu);
-    QK_HIP(hipSetDevice(d->dev));
-    const size_t ns = align64(size_t(n));
-    QK_HIP(d->recs.ensure(ns * sizeof(OptionRec)));
-    QK_HIP(d->ntiles.ensure(ns * 4));
-    QK_HIP(d->tile_start.ensure((ns + 1) * 4));
-    QK_HIP(d->counters.ensure(64));
-    QK_HIP(d->pin_in.ensure(64));
-    QK_HIP(d->in.ensure(64));
-    QK_HIP(d->pin_out.ensure(64));
-    std::memcpy(d->pin_in.p, &n, 4);
-    QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, 4, hipMemcpyHostToDevice, d->stream));
-    const int32_t* n_dev = static_cast<const int32_t*>(d->in.p);
+int32_t mc_device_locked(Device* d, const BatchIn& in, const int32_t* n_dev, double* out_price_dev,
+                         double* out_se_dev, std::string* err) {
     // tile count is only known on the device: prepare + scan, read it back,
     // size the partials, then simulate.
     OptionRec* recs = static_cast<OptionRec*>(d->recs.p);
@@ -506,6 +424,38 @@ int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
     QK_HIP(hipEventRecord(d->ev[2], d->stream));
     QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, out_price_dev, out_se_dev));
     QK_HIP(hipEventRecord(d->ev[3], d->stream));
+    return QK_OK;
+}
+
+int32_t ensure_mc_capacity(Device* d, int32_t n, std::string* err) {
+    const size_t ns = align64(size_t(n));
+    QK_HIP(d->recs.ensure(ns * sizeof(OptionRec)));
+    QK_HIP(d->ntiles.ensure(ns * 4));
+    QK_HIP(d->tile_start.ensure((ns + 1) * 4));
+    QK_HIP(d->counters.ensure(64));
+    QK_HIP(d->pin_out.ensure(64));
+    return QK_OK;
+}
+
+int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
+                            double* out_se_dev, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    if (get_rng_mode() != QK_RNG_PHILOX) {
+        if (err) *err = "device-resident batches support QK_RNG_PHILOX only";
+        return QK_ERR_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+    int32_t rc = ensure_mc_capacity(d, n, err);
+    if (rc != QK_OK) return rc;
+    QK_HIP(d->pin_in.ensure(64));
+    QK_HIP(d->in.ensure(64));
+    std::memcpy(d->pin_in.p, &n, 4);
+    QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, 4, hipMemcpyHostToDevice, d->stream));
+    rc = mc_device_locked(d, in, static_cast<const int32_t*>(d->in.p), out_price_dev, out_se_dev,
+                          err);
+    if (rc != QK_OK) return rc;
     QK_HIP(hipStreamSynchronize(d->stream));
     record_timing(d);
     return QK_OK;

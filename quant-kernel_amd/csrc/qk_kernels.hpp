@@ -20,6 +20,8 @@ struct BatchIn {
     const uint64_t* seed;
     int32_t uniform_method;
     int32_t method_is_model;
+    const uint8_t* active;   // optional: elements with active[i] == 0 are skipped (NaN)
+    int32_t paths_override;  // > 0: every element uses this path count
 };
 
 // Path precision / normal source of a simulation launch.

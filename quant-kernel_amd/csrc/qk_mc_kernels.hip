@@ -136,11 +136,11 @@ __global__ __launch_bounds__(256) void k_prepare(BatchIn in, const int32_t* __re
         o.sigma = in.sigma ? in.sigma[i] : 0.0;
         o.rho = in.rho ? in.rho[i] : 0.0;
         o.ot = in.ot[i];
-        o.paths = in.paths[i];
+        o.paths = in.paths_override > 0 ? in.paths_override : in.paths[i];
         o.steps = in.steps ? in.steps[i] : 1;
         o.seed = in.seed[i];
         OptionRec rec;
-        if (m < 0) {
+        if (m < 0 || (in.active && !in.active[i])) {
             prepare_option(o, 1, rec);  // fills defaults
             rec.family = FAM_NONE;
             rec.closed = qnan_d();

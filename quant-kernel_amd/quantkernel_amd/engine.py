@@ -115,6 +115,19 @@ _SIGNATURES = {
     "qk_device_info": (_I, [ct.c_char_p, _I, _PI]),
     "qk_mc_philox_probe": (_I, [_U, ct.c_uint32, ct.c_uint32, ct.c_uint32, _I,
                                 ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_float)]),
+    "qk_sur_param_count": (_I, []),
+    "qk_sur_init": (_I, [_U]),
+    "qk_sur_train": (_I, [_PD, _PD, _I, _I, _I, _D, _D, _U, _PD]),
+    "qk_sur_predict": (_I, [_PD, _I, _PD]),
+    "qk_sur_gradients": (_I, [_PD, _PD, _I, ct.POINTER(ct.c_float), _PD]),
+    "qk_sur_get_params": (_I, [ct.POINTER(ct.c_float)]),
+    "qk_sur_set_params": (_I, [ct.POINTER(ct.c_float)]),
+    "qk_sur_save": (_I, [ct.c_char_p]),
+    "qk_sur_load": (_I, [ct.c_char_p]),
+    "qk_sur_configure": (_I, [_I, _I, _D]),
+    "qk_sur_heston_price_batch": (_I, [_PD] * 10 + [_PI, _PI, _PI, _PU, _I, _PD, _PI]),
+    "qk_sur_heston_price_batch_device": (_I, [ct.c_void_p] * 14 + [_I, ct.c_void_p, ct.c_void_p]),
+    "qk_sur_last_timing": (_I, [_PD, _PD]),
 }
 
 
@@ -206,6 +219,97 @@ class QuantKernel:
                                                  normals.ctypes.data_as(ct.POINTER(ct.c_float))),
                     "qk_mc_philox_probe")
         return words.reshape(count, 4), normals.reshape(count, 4)
+
+    # ---- Heston surrogate (ABI 2.13) --------------------------------------
+    def sur_param_count(self) -> int:
+        return int(self._lib.qk_sur_param_count())
+
+    def sur_init(self, seed: int = 0) -> None:
+        self._check(self._lib.qk_sur_init(int(seed)), "qk_sur_init")
+
+    @staticmethod
+    def _features(f) -> np.ndarray:
+        f = np.ascontiguousarray(np.asarray(f, dtype=np.float64))
+        if f.ndim != 2 or f.shape[1] != 8:
+            raise ValueError("features must be an (n, 8) array")
+        return f
+
+    def sur_train(self, features, targets, epochs=10, batch=8192, lr=1e-3, lr_final=1e-5, seed=0):
+        f = self._features(features)
+        t = self._as_f64_array(targets, "targets")
+        self._check_same_length(f.shape[0], targets=t)
+        loss = np.empty(int(epochs))
+        self._check(self._lib.qk_sur_train(f.ctypes.data_as(_PD), t.ctypes.data_as(_PD), f.shape[0],
+                                           int(epochs), int(batch), float(lr), float(lr_final), int(seed),
+                                           loss.ctypes.data_as(_PD)), "qk_sur_train")
+        return loss
+
+    def sur_predict(self, features) -> np.ndarray:
+        f = self._features(features)
+        out = np.empty(f.shape[0])
+        self._check(self._lib.qk_sur_predict(f.ctypes.data_as(_PD), f.shape[0], out.ctypes.data_as(_PD)),
+                    "qk_sur_predict")
+        return out
+
+    def sur_gradients(self, features, targets):
+        f = self._features(features)
+        t = self._as_f64_array(targets, "targets")
+        g = np.empty(self.sur_param_count(), dtype=np.float32)
+        loss = ct.c_double()
+        self._check(self._lib.qk_sur_gradients(f.ctypes.data_as(_PD), t.ctypes.data_as(_PD), f.shape[0],
+                                               g.ctypes.data_as(ct.POINTER(ct.c_float)), ct.byref(loss)),
+                    "qk_sur_gradients")
+        return g, loss.value
+
+    def sur_get_params(self) -> np.ndarray:
+        p = np.empty(self.sur_param_count(), dtype=np.float32)
+        self._check(self._lib.qk_sur_get_params(p.ctypes.data_as(ct.POINTER(ct.c_float))), "qk_sur_get_params")
+        return p
+
+    def sur_set_params(self, params) -> None:
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
+        if p.shape != (self.sur_param_count(),):
+            raise ValueError("wrong parameter vector length")
+        self._check(self._lib.qk_sur_set_params(p.ctypes.data_as(ct.POINTER(ct.c_float))), "qk_sur_set_params")
+
+    def sur_save(self, path) -> None:
+        self._check(self._lib.qk_sur_save(str(path).encode()), "qk_sur_save")
+
+    def sur_load(self, path) -> None:
+        self._check(self._lib.qk_sur_load(str(path).encode()), "qk_sur_load")
+
+    def sur_configure(self, spot_every=128, spot_paths=8192, spot_k_se=3.0) -> None:
+        self._check(self._lib.qk_sur_configure(int(spot_every), int(spot_paths), float(spot_k_se)),
+                    "qk_sur_configure")
+
+    def sur_heston_price_batch(self, spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type,
+                               paths, steps, seed):
+        f = [self._as_f64_array(x, nm) for x, nm in
+             zip((spot, strike, t, r, q, v0, kappa, theta, sigma, rho),
+                 ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho"))]
+        n = f[0].shape[0]
+        ot, p, st, s = (self._as_i32_array(option_type, "option_type"), self._as_i32_array(paths, "paths"),
+                        self._as_i32_array(steps, "steps"), self._as_u64_array(seed, "seed"))
+        self._check_same_length(n, option_type=ot, paths=p, steps=st, seed=s)
+        out = np.empty(n)
+        src = np.empty(n, dtype=np.int32)
+        self._check(self._lib.qk_sur_heston_price_batch(*[self._ptr(a) for a in f + [ot, p, st, s]], n,
+                                                        out.ctypes.data_as(_PD), src.ctypes.data_as(_PI)),
+                    "qk_sur_heston_price_batch")
+        return out, src
+
+    def sur_heston_price_batch_device(self, n: int, ptrs: dict, out_price_ptr: int, out_source_ptr: int):
+        order = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho", "option_type",
+                 "paths", "steps", "seed")
+        args = [ct.c_void_p(int(ptrs[k])) for k in order]
+        self._check(self._lib.qk_sur_heston_price_batch_device(*args, int(n), ct.c_void_p(int(out_price_ptr)),
+                                                               ct.c_void_p(int(out_source_ptr))),
+                    "qk_sur_heston_price_batch_device")
+
+    def sur_last_timing(self) -> tuple[float, float]:
+        a, b = ct.c_double(), ct.c_double()
+        self._check(self._lib.qk_sur_last_timing(ct.byref(a), ct.byref(b)), "qk_sur_last_timing")
+        return a.value, b.value
 
     # ---- helpers (engine.py:99-124, 163-194) -----------------------------
     @staticmethod
