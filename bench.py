@@ -15,10 +15,17 @@ Alongside, in the same run:
   * the reference's CPU batch entry point (a build of its sources, -O3
     -fopenmp) on the first queries, same paths and steps.
 
---workload c1 runs the plain MC configuration instead: 16384 Heston calls x
-65536 paths x 256 steps.
+Other workloads (reported with the same fields; not the headline line):
+  c0  4096 GBM calls/puts, standard MC (antithetic terminal), 100000 paths;
+  c1  plain MC: 16384 Heston calls x 65536 paths x 256 steps;
+  c2  50000 uneven mixed options (GBM Euler / Milstein / Heston / local vol),
+      paths log-uniform in [4096, 1048576], steps log-uniform in [16, 512];
+  c3  1024 American puts, Longstaff-Schwartz, 100000 paths x 50 dates,
+      basis {1, x, x^2, x^3}, x = S/K.
+c0 and c3 go through the host-memory entry points (their inputs are 64 and
+100 bytes per option); c1, c2 and c4 use the device-resident ones.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c1]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c0|c1|c2|c3|c4]
 """
 import argparse
 import ctypes as ct
@@ -159,6 +166,174 @@ def bench_c1(args, qk, qka, dev, name, cus):
     return res
 
 
+def ref_fn(name):
+    """A batch entry point of the reference build (or the oracle port)."""
+    ref = ROOT / "oracle" / "_ref" / "libqkref.so"
+    if ref.exists():
+        return "reference", getattr(ct.CDLL(str(ref)), name)
+    return None, None
+
+
+def time_cpu(fn, arrays, n_args, k, threads):
+    PD, PI, PU = ct.POINTER(ct.c_double), ct.POINTER(ct.c_int32), ct.POINTER(ct.c_uint64)
+    conv = []
+    for a in arrays:
+        a = np.ascontiguousarray(a[:k])
+        conv.append(a)
+    ptrs = [a.ctypes.data_as(PD if a.dtype == np.float64 else PI if a.dtype == np.int32 else PU) for a in conv]
+    out = np.empty(k)
+    t0 = time.perf_counter()
+    fn(*ptrs, ct.c_int32(k), out.ctypes.data_as(PD))
+    return time.perf_counter() - t0
+
+
+def cpu_sample(fn, arrays, n, threads, budget_s=10.0, per=1):
+    k = min(n, max(per, threads))
+    dt = time_cpu(fn, arrays, None, k, threads)
+    if dt < budget_s / 3 and k < n:
+        k = min(n, max(k, int(k * budget_s / max(dt, 1e-3))))
+        dt = time_cpu(fn, arrays, None, k, threads)
+    return k, dt
+
+
+def bench_c0(args, qk, qka, dev, name, cus):
+    from quantkernel_amd import workloads as wl
+    n = args.options or 4096
+    d = wl.config0(n)
+    a = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths")]
+    for _ in range(args.warmup):
+        qk.standard_monte_carlo_price_batch(*a, d["seed"])
+    tot, sims = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p, se = qk.standard_monte_carlo_price_batch(*a, d["seed"], return_stderr=True)
+        tt, sm = qk.last_timing()
+        tot.append(tt)
+        sims.append(sm)
+        log(f"step done: device {tt:.3f} ms, terminal kernel {sm:.3f} ms")
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    sim_ms = float(np.mean(sims))
+    flops = float(n) * 100000 * 11
+    ach = flops / (sim_ms * 1e-3) / 1e12
+    res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
+               config={"workload": f"C0: {n} GBM European calls/puts, standard MC (antithetic terminal), 100000 paths "
+                                   "(Philox, FP32 paths, FP64 sums); host-memory entry point", "options": n,
+                       "device_ms": float(np.mean(tot)), "device": name, "cus": cus,
+                       "median_stderr_over_spot": float(np.median(se / d["spot"]))},
+               roofline={"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                         "kernel": "qk::k_sim_terminal<float, PHILOX>", "kernel_ms": sim_ms, "flops_per_launch": flops})
+    if not args.no_cpu:
+        kind, fn = ref_fn("qk_mcm_standard_monte_carlo_price_batch")
+        if fn:
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            k, dt = cpu_sample(fn, a + [d["seed"]], n, threads)
+            res["cpu_baseline"] = {"value": k / dt, "unit": "prices/s", "cores": threads, "kind": kind,
+                                   "sample": f"first {k} options, reference standard MC batch, {threads} threads"}
+    return res
+
+
+def bench_c2(args, qk, qka, dev, name, cus):
+    import torch
+    from quantkernel_amd import workloads as wl
+    n = args.options or 50000
+    d = wl.config2(n)
+    t = to_device(d, dev)
+    ptrs = {k: v.data_ptr() for k, v in t.items()}
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    se = torch.empty(n, dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+    torch.cuda.synchronize()
+    sims = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+        sims.append(qk.last_timing()[1])
+        log(f"step done: sim kernel {sims[-1]:.2f} ms")
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    sim_ms = float(np.mean(sims))
+    per_step = {0: 14, 1: 18, 2: FLOPS_PER_PATH_STEP, 3: 15}
+    ps = d["paths"].astype(np.float64) * d["steps"]
+    flops = float(sum(ps[d["model"] == m].sum() * f for m, f in per_step.items()))
+    ach = flops / (sim_ms * 1e-3) / 1e12
+    sev = se.cpu().numpy() / d["spot"]
+    res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
+               config={"workload": f"C2: {n} uneven options, 25% each GBM-Euler / GBM-Milstein / Heston-Euler / "
+                                   "constant local vol; paths log-uniform [4096, 1048576], steps log-uniform "
+                                   "[16, 512]; one persistent launch (device-resident entry point)",
+                       "options": n, "path_steps": float(ps.sum()), "device": name, "cus": cus,
+                       "median_stderr_over_spot": float(np.median(sev)), "max_stderr_over_spot": float(np.max(sev))},
+               roofline={"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                         "kernel": "qk::k_sim_stepwise<float, PHILOX>", "kernel_ms": sim_ms, "flops_per_launch": flops})
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        total_t, total_k, kind = 0.0, 0, None
+        names = {0: "qk_mcm_euler_maruyama_price_batch", 1: "qk_mcm_milstein_price_batch",
+                 3: "qk_mcm_local_vol_monte_carlo_price_batch"}
+        for m in (0, 1, 2, 3):
+            idx = np.nonzero(d["model"] == m)[0]
+            sub = {k: v[idx] for k, v in d.items()}
+            if m == 2:
+                kind, fn = ref_fn("qk_mcm_heston_monte_carlo_price_batch")
+                arrays = [sub[k] for k in KEYS] + [sub["option_type"], sub["paths"], sub["steps"], sub["seed"]]
+            else:
+                kind, fn = ref_fn(names[m])
+                arrays = [sub[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths", "steps",
+                                           "seed")]
+            if not fn:
+                break
+            k, dt = cpu_sample(fn, arrays, len(idx), threads, budget_s=4.0)
+            total_t += dt / k * len(idx)  # extrapolated time for this model's share
+            total_k += k
+        if kind:
+            res["cpu_baseline"] = {"value": n / total_t, "unit": "prices/s", "cores": threads, "kind": kind,
+                                   "sample": f"{total_k} options (a prefix of each model's share), reference batch "
+                                             f"entry points, {threads} threads; time extrapolated per model share"}
+    return res
+
+
+def bench_c3(args, qk, qka, dev, name, cus):
+    from quantkernel_amd import workloads as wl
+    n = args.options or 1024
+    d = wl.config3(n)
+    a = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths", "steps")]
+    for _ in range(args.warmup):
+        qk.longstaff_schwartz_price_batch(*a, d["seed"], basis=1)
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p, se = qk.longstaff_schwartz_price_batch(*a, d["seed"], basis=1, return_stderr=True)
+        times.append(qk.last_timing()[0])
+        log(f"step done: device {times[-1]:.2f} ms")
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    dev_ms = float(np.mean(times))
+    # algorithmic HBM bytes: write S (4 B) once, then per regression date read S_j (4 B)
+    # and read+write the FP32 cashflow (8 B): 4 + 12 * (dates - 1) bytes per path
+    paths, dates = 100000, 50
+    byts = float(n) * paths * (4 * dates + 12 * (dates - 1))
+    ach = byts / (dev_ms * 1e-3) / 1e9
+    res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
+               config={"workload": f"C3: {n} American puts, Longstaff-Schwartz, {paths} paths x {dates} dates, basis "
+                                   "{1,x,x^2,x^3}, x=S/K (Philox, FP32 paths, FP64 normal equations); host entry",
+                       "options": n, "device": name, "cus": cus,
+                       "median_stderr_over_spot": float(np.median(se / d["spot"]))},
+               roofline={"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
+                         "traffic": None, "kernel": "qk::k_lsmc<float, PHILOX, 4>", "kernel_ms": dev_ms,
+                         "bytes_per_launch": byts})
+    if not args.no_cpu:
+        kind, fn = ref_fn("qk_mcm_longstaff_schwartz_price_batch")
+        if fn:
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            k, dt = cpu_sample(fn, a + [d["seed"]], n, threads)
+            res["cpu_baseline"] = {"value": k / dt, "unit": "prices/s", "cores": threads, "kind": kind,
+                                   "sample": f"first {k} options, reference LSMC batch (its {{1,S,S^2}} basis), "
+                                             f"{threads} threads"}
+    return res
+
+
 def bench_c4(args, qk, qka, dev, name, cus):
     import torch
     from quantkernel_amd import surrogate as sur
@@ -255,7 +430,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c4", "c1"], default=None)
+    ap.add_argument("--workload", choices=["c0", "c1", "c2", "c3", "c4"], default=None)
     ap.add_argument("--options", type=int, default=0)
     ap.add_argument("--weights", default=str(ROOT / "quant-kernel_amd" / "quantkernel_amd" / "heston_surrogate.qkw"))
     ap.add_argument("--accuracy-queries", type=int, default=1024)
@@ -273,7 +448,8 @@ def main():
     name, cus = qk.device_info()
     dev = torch.device("cuda:0")
     workload = args.workload or ("c4" if Path(args.weights).exists() else "c1")
-    res = (bench_c4 if workload == "c4" else bench_c1)(args, qk, qka, dev, name, cus)
+    res = {"c0": bench_c0, "c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[workload](
+        args, qk, qka, dev, name, cus)
     line = {"metric": METRIC, "value": res["value"], "unit": "prices/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": res["dtype"], "data": "synthetic", "config": res["config"],

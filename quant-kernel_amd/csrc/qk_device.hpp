@@ -88,7 +88,7 @@ struct Device {
     std::mutex mu;
 
     DevBuf in, recs, ntiles, tile_start, counters, partials, out, stream_buf, stream_off;
-    DevBuf lsmc_path, lsmc_cf, lsmc_recs;
+    DevBuf lsmc_path, lsmc_cf, lsmc_recs, lsmc_part, lsmc_beta;
     PinBuf pin_in, pin_out;
 
     double ms_total = -1.0, ms_sim = -1.0;

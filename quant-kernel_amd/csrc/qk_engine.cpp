@@ -518,10 +518,10 @@ int32_t run_lsmc_batch(const double* spot, const double* strike, const double* t
     QK_HIP(hipEventRecord(d->ev[1], d->stream));
     size_t first = 0;
     while (first < all.size()) {
-        // chunk [first, last)
+        // chunk [first, last): bounded by the memory budget and the grid's y extent
         size_t last = first;
         int64_t path_elems = 0, cf_elems = 0, draws = 0;
-        while (last < all.size()) {
+        while (last < all.size() && last - first < 65535) {
             const int64_t pe = int64_t(all[last].paths) * all[last].steps;
             const int64_t bytes = (path_elems + pe + cf_elems + all[last].paths) * int64_t(esz);
             if (last > first && bytes > budget) break;
@@ -534,6 +534,13 @@ int32_t run_lsmc_batch(const double* spot, const double* strike, const double* t
             ++last;
         }
         const int32_t count = int32_t(last - first);
+        int32_t tiles_max = 1, max_steps = 1;
+        for (size_t k = first; k < last; ++k) {
+            tiles_max = std::max(tiles_max, (all[k].paths + kLsmcTile - 1) / kLsmcTile);
+            max_steps = std::max(max_steps, all[k].steps);
+        }
+        QK_HIP(d->lsmc_part.ensure(size_t(count) * tiles_max * lsmc_record_doubles(basis) * 8));
+        QK_HIP(d->lsmc_beta.ensure(size_t(count) * 5 * 8));
         QK_HIP(d->lsmc_path.ensure(size_t(path_elems) * esz));
         QK_HIP(d->lsmc_cf.ensure(size_t(cf_elems) * esz));
         QK_HIP(d->lsmc_recs.ensure(size_t(count) * sizeof(LsmcRec)));
@@ -575,8 +582,10 @@ int32_t run_lsmc_batch(const double* spot, const double* strike, const double* t
             dstream = d->stream_buf.p;
         }
         QK_HIP(launch_lsmc(d->stream, sim_mode, basis,
-                           static_cast<const LsmcRec*>(d->lsmc_recs.p), count, d->lsmc_path.p,
-                           d->lsmc_cf.p, dstream, dprice, dse));
+                           static_cast<const LsmcRec*>(d->lsmc_recs.p), count, tiles_max,
+                           max_steps, d->lsmc_path.p, d->lsmc_cf.p, dstream,
+                           static_cast<double*>(d->lsmc_part.p),
+                           static_cast<double*>(d->lsmc_beta.p), dprice, dse));
         QK_HIP(hipStreamSynchronize(d->stream));  // pinned recs / host stream reused next chunk
         first = last;
     }

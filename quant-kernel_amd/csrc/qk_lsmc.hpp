@@ -8,6 +8,10 @@
 
 namespace qk {
 
+constexpr int kLsmcBlock = 256;    // threads per workgroup
+constexpr int kLsmcPerLane = 16;   // paths per lane per tile
+constexpr int kLsmcTile = kLsmcBlock * kLsmcPerLane;
+
 // One option of an LSMC launch (host-prepared).
 struct alignas(16) LsmcRec {
     int32_t out_index;  // position in the caller's output arrays
@@ -30,9 +34,15 @@ int32_t run_lsmc_batch(const double* spot, const double* strike, const double* t
                        int32_t basis, int32_t n, double* out_price, double* out_se,
                        std::string* err);
 
-// Kernel launcher (qk_lsmc.hip).  mode is a SimMode, basis QK_LSMC_BASIS_*.
+// Doubles per (option, tile) partial record of a basis.
+int lsmc_record_doubles(int32_t basis);
+
+// Launches forward, (solve, sweep) x (max_steps - 1) and final for one chunk
+// (qk_lsmc.hip).  mode is a SimMode, basis QK_LSMC_BASIS_*; part holds
+// count * tiles_max records, beta count * 5 doubles.
 hipError_t launch_lsmc(hipStream_t st, int32_t mode, int32_t basis, const LsmcRec* recs,
-                       int32_t count, void* pathbuf, void* cfbuf, const void* stream,
+                       int32_t count, int32_t tiles_max, int32_t max_steps, void* pathbuf,
+                       void* cfbuf, const void* stream, double* part, double* beta,
                        double* out_price, double* out_se);
 
 }  // namespace qk
