@@ -87,7 +87,7 @@ struct Device {
     hipEvent_t ev_sur[3] = {};  // surrogate pricing phases
     std::mutex mu;
 
-    DevBuf in, recs, ntiles, tile_start, counters, partials, out, stream_buf, stream_off;
+    DevBuf in, recs, ntiles, tile_start, counters, partials, out, stream_buf, stream_off, scan;
     DevBuf lsmc_path, lsmc_cf, lsmc_recs, lsmc_part, lsmc_beta;
     PinBuf pin_in, pin_out;
 

@@ -106,7 +106,7 @@ size_t align64(size_t x) { return (x + 63) & ~size_t(63); }
 
 // Launch prepare -> scan -> simulate -> finalize on d's stream.  The caller
 // holds d->mu.  `family` selects the simulation kernel.
-int32_t launch_pipeline(Device* d, const BatchIn& in, const int32_t* n_dev,
+int32_t launch_pipeline(Device* d, const BatchIn& in, int32_t n, const int32_t* n_dev,
                         const int64_t* stream_off, int32_t family, int32_t mode,
                         const void* stream, double* out_price, double* out_se,
                         std::string* err) {
@@ -119,7 +119,7 @@ int32_t launch_pipeline(Device* d, const BatchIn& in, const int32_t* n_dev,
     QK_HIP(hipEventRecord(d->ev[0], d->stream));
     QK_HIP(launch_prepare(d->stream, in, n_dev, stream_off, tile_units_terminal(d->shape_term),
                           tile_units_stepwise(d->shape_step), recs, ntiles));
-    QK_HIP(launch_scan(d->stream, n_dev, ntiles, recs, tstart, ctr));
+    QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p));
     QK_HIP(hipEventRecord(d->ev[1], d->stream));
     QK_HIP(launch_simulate(d->stream, family, mode, shp, recs, tstart, n_dev, ctr, parts, stream));
     QK_HIP(hipEventRecord(d->ev[2], d->stream));
@@ -311,6 +311,7 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
     QK_HIP(d->ntiles.ensure(ns * 4));
     QK_HIP(d->tile_start.ensure((ns + 1) * 4));
     QK_HIP(d->counters.ensure(64));
+    QK_HIP(d->scan.ensure((ns / 1024 + 2) * 8));
     QK_HIP(d->partials.ensure(size_t(std::max<int64_t>(tiles, 1)) * sizeof(Partial)));
     QK_HIP(d->out.ensure(ns * 16 + 64));
     QK_HIP(d->pin_out.ensure(ns * 16 + 64));
@@ -375,7 +376,7 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
         d_stream = d->stream_buf.p;
     }
 
-    int32_t rc = launch_pipeline(d, bi, n_dev, d_off, family, sim_mode, d_stream, dprice, dse, err);
+    int32_t rc = launch_pipeline(d, bi, n, n_dev, d_off, family, sim_mode, d_stream, dprice, dse, err);
     if (rc != QK_OK) return rc;
     // outputs + the scan's overflow flag
     QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, ns * 16, hipMemcpyDeviceToHost, d->stream));
@@ -394,8 +395,8 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
     return QK_OK;
 }
 
-int32_t mc_device_locked(Device* d, const BatchIn& in, const int32_t* n_dev, double* out_price_dev,
-                         double* out_se_dev, std::string* err) {
+int32_t mc_device_locked(Device* d, const BatchIn& in, int32_t n, const int32_t* n_dev,
+                         double* out_price_dev, double* out_se_dev, std::string* err) {
     // tile count is only known on the device: prepare + scan, read it back,
     // size the partials, then simulate.
     OptionRec* recs = static_cast<OptionRec*>(d->recs.p);
@@ -405,7 +406,7 @@ int32_t mc_device_locked(Device* d, const BatchIn& in, const int32_t* n_dev, dou
     QK_HIP(hipEventRecord(d->ev[0], d->stream));
     QK_HIP(launch_prepare(d->stream, in, n_dev, nullptr, tile_units_terminal(d->shape_term),
                           tile_units_stepwise(d->shape_step), recs, ntiles));
-    QK_HIP(launch_scan(d->stream, n_dev, ntiles, recs, tstart, ctr));
+    QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p));
     QK_HIP(hipMemcpyAsync(d->pin_out.p, ctr, 16, hipMemcpyDeviceToHost, d->stream));
     QK_HIP(hipStreamSynchronize(d->stream));
     const int32_t* hc = static_cast<const int32_t*>(d->pin_out.p);
@@ -434,6 +435,7 @@ int32_t ensure_mc_capacity(Device* d, int32_t n, std::string* err) {
     QK_HIP(d->tile_start.ensure((ns + 1) * 4));
     QK_HIP(d->counters.ensure(64));
     QK_HIP(d->pin_out.ensure(64));
+    QK_HIP(d->scan.ensure((ns / 1024 + 2) * 8));
     return QK_OK;
 }
 
@@ -453,7 +455,7 @@ int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
     QK_HIP(d->in.ensure(64));
     std::memcpy(d->pin_in.p, &n, 4);
     QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, 4, hipMemcpyHostToDevice, d->stream));
-    rc = mc_device_locked(d, in, static_cast<const int32_t*>(d->in.p), out_price_dev, out_se_dev,
+    rc = mc_device_locked(d, in, n, static_cast<const int32_t*>(d->in.p), out_price_dev, out_se_dev,
                           err);
     if (rc != QK_OK) return rc;
     QK_HIP(hipStreamSynchronize(d->stream));

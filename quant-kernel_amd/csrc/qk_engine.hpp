@@ -36,8 +36,8 @@ struct Device;
 // The device MC pipeline on device-resident inputs; the caller holds d->mu
 // and has sized the per-option buffers (ensure_mc_capacity).  Synchronises
 // once to size the tile partials; leaves the finalize queued on d->stream.
-int32_t mc_device_locked(Device* d, const BatchIn& in, const int32_t* n_dev, double* out_price_dev,
-                         double* out_se_dev, std::string* err);
+int32_t mc_device_locked(Device* d, const BatchIn& in, int32_t n, const int32_t* n_dev,
+                         double* out_price_dev, double* out_se_dev, std::string* err);
 int32_t ensure_mc_capacity(Device* d, int32_t n, std::string* err);
 // Packs a host batch into the device staging buffer (one H2D copy, queued).
 int32_t stage_host_batch(Device* d, const HostBatch& hb, int32_t n, BatchIn* bi,

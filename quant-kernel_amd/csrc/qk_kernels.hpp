@@ -46,8 +46,9 @@ inline int32_t tile_units_terminal(const SimShape& s) { return kBlock * s.lane_u
 hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev,
                           const int64_t* stream_off, int32_t tu_terminal, int32_t tu_stepwise,
                           OptionRec* recs, int32_t* ntiles);
-hipError_t launch_scan(hipStream_t st, const int32_t* n_dev, const int32_t* ntiles,
-                       OptionRec* recs, int32_t* tile_start, int32_t* counters);
+// scratch: (n + 1)/1024 + 1 int64 block sums
+hipError_t launch_scan(hipStream_t st, int32_t n, const int32_t* n_dev, const int32_t* ntiles,
+                       OptionRec* recs, int32_t* tile_start, int32_t* counters, void* scratch);
 hipError_t launch_simulate(hipStream_t st, int32_t family, int32_t mode, const SimShape& shape,
                            const OptionRec* recs, const int32_t* tile_start,
                            const int32_t* n_dev, int32_t* counters, Partial* partials,

@@ -156,43 +156,75 @@ __global__ __launch_bounds__(256) void k_prepare(BatchIn in, const int32_t* __re
     }
 }
 
-// Exclusive scan of the tile counts by one workgroup of 1024 threads; each
-// thread owns a contiguous chunk.  Also resets the tile counter.
-__global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ n_dev,
-                                               const int32_t* __restrict__ ntiles,
-                                               OptionRec* __restrict__ recs,
-                                               int32_t* __restrict__ tile_start,
-                                               int32_t* __restrict__ counters) {
-    __shared__ long long part[1024];
-    const int32_t n = *n_dev;
-    const int32_t tid = threadIdx.x;
-    const int32_t per = (n + 1023) / 1024;
-    const int32_t beg = min(n, tid * per), end = min(n, beg + per);
-    long long local = 0;
-    for (int32_t i = beg; i < end; ++i) local += ntiles[i];
-    part[tid] = local;
+// Exclusive scan of the tile counts in three passes: per-1024-element block
+// totals, one workgroup scanning the block totals (and resetting the tile
+// counter), then per-block offsets.  Sums are 64-bit; a total beyond 2^31-1
+// sets the overflow flag and schedules no tile.
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ long long block_inclusive_scan(long long v, long long* sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const long long v = tid >= off ? part[tid - off] : 0;
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        const long long add = tid >= off ? sh[tid - off] : 0;
         __syncthreads();
-        part[tid] += v;
+        sh[tid] += add;
         __syncthreads();
     }
-    const long long total = part[1023];
-    const bool overflow = total > 0x7fffffffLL;
-    long long run = part[tid] - local;
-    for (int32_t i = beg; i < end; ++i) {
-        const int32_t v = overflow ? 0 : int32_t(run);
-        tile_start[i] = v;
-        recs[i].tile_first = v;
-        run += ntiles[i];
+    return sh[tid];
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(const int32_t* __restrict__ n_dev,
+                                                            const int32_t* __restrict__ ntiles,
+                                                            long long* __restrict__ block_sum) {
+    __shared__ long long sh[kScanBlock];
+    const int32_t n = *n_dev;
+    const int32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    const long long v = i < n ? ntiles[i] : 0;
+    const long long inc = block_inclusive_scan(v, sh);
+    if (threadIdx.x == kScanBlock - 1) block_sum[blockIdx.x] = inc;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_top(int32_t nblocks,
+                                                         long long* __restrict__ block_sum,
+                                                         int32_t* __restrict__ counters) {
+    __shared__ long long sh[kScanBlock];
+    long long carry = 0;
+    for (int32_t base = 0; base < nblocks; base += kScanBlock) {
+        const int32_t i = base + threadIdx.x;
+        const long long v = i < nblocks ? block_sum[i] : 0;
+        const long long inc = block_inclusive_scan(v, sh);
+        if (i < nblocks) block_sum[i] = carry + inc - v;  // exclusive block offset
+        carry += sh[kScanBlock - 1];
+        __syncthreads();
     }
-    if (tid == 0) {
-        tile_start[n] = overflow ? 0 : int32_t(total);
+    if (threadIdx.x == 0) {
+        const bool overflow = carry > 0x7fffffffLL;
         counters[0] = 0;
-        counters[1] = overflow ? 0 : int32_t(total);
+        counters[1] = overflow ? 0 : int32_t(carry);
         counters[2] = overflow ? 1 : 0;
     }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int32_t* __restrict__ n_dev,
+                                                           const int32_t* __restrict__ ntiles,
+                                                           const long long* __restrict__ block_off,
+                                                           const int32_t* __restrict__ counters,
+                                                           OptionRec* __restrict__ recs,
+                                                           int32_t* __restrict__ tile_start) {
+    __shared__ long long sh[kScanBlock];
+    const int32_t n = *n_dev;
+    const int32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    const long long v = i < n ? ntiles[i] : 0;
+    const long long inc = block_inclusive_scan(v, sh);
+    const bool overflow = counters[2] != 0;
+    if (i < n) {
+        const int32_t off = overflow ? 0 : int32_t(block_off[blockIdx.x] + inc - v);
+        tile_start[i] = off;
+        recs[i].tile_first = off;
+    }
+    if (i == n) tile_start[n] = counters[1];
 }
 
 // ----------------------------------------------------------------------------
@@ -295,7 +327,7 @@ __device__ __forceinline__ void heston_step(const HestonC<R>& k, R& x, R& v, R z
     x = q_fma(k.c * sv, z1, x + drift);
     x = q_max(x, k.xf);
     const R mix = q_fma(k.rho, z1, k.rc * z2);
-    const R vn = q_fma(k.sg * sv, mix, q_fma(-k.kdt, v, v + k.kth));
+    const R vn = q_fma(k.sg * sv, mix, q_fma(R(1) - k.kdt, v, k.kth));
     v = q_max(vn, R(0));
 }
 
@@ -314,6 +346,47 @@ __device__ __forceinline__ R heston_path(const Src& src, uint32_t p, int32_t ste
         heston_step(k, x, v, z.z[0], z.z[1]);
     }
     return x;
+}
+
+// Production Heston path: the same scheme and the same Philox draws as
+// heston_path, rearranged to issue fewer instructions.  With Box-Muller
+// radius r = sqrt(2 ln 2) * sqrt(-log2 u1) and angle theta,
+//   z1              = r cos(theta)
+//   rho z1 + rho' z2 = r cos(theta - phi),   cos phi = rho, sin phi = rho'
+// and the log-spot is kept in units of c sqrt(2 ln 2) (c = sqrt(dt) log2e),
+// so per step: y <- max(y + (a - h v)/cK + sqrt(v) r' cos(theta), floor/cK),
+// v <- max(0, (1 - kappa dt) v + kappa theta dt + sigma sqrt(dt) sqrt(2 ln 2)
+// sqrt(v) r' cos(theta - phi)), with r' = sqrt(-log2 u1).
+struct HestonPx {
+    float a, h, omk, kth, sgk, phi, yf, v0, ck;
+};
+
+__device__ __forceinline__ void heston_px_step(const HestonPx& k, float& y, float& v, uint32_t wa,
+                                               uint32_t wb) {
+    const float u1 = __builtin_fmaf(float(wa), 0x1p-32f, 0x1p-33f);
+    const float rev = __builtin_bit_cast(float, (wb & 0x007FFFFFu) | 0x3F800000u);
+    const float rr = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1));
+    const float c1 = __builtin_amdgcn_cosf(rev);
+    const float c2 = __builtin_amdgcn_cosf(rev - k.phi);
+    const float t = __builtin_amdgcn_sqrtf(v) * rr;
+    y = __builtin_fmaxf(__builtin_fmaf(t, c1, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
+    v = __builtin_fmaxf(__builtin_fmaf(k.sgk * t, c2, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
+}
+
+__device__ __forceinline__ float heston_path_px(uint32_t p, int32_t steps, const HestonPx& k,
+                                                uint32_t k0, uint32_t k1) {
+    float y = 0.0f, v = k.v0;
+    const int32_t nb = steps >> 1;
+    for (int32_t b = 0; b < nb; ++b) {
+        const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
+        heston_px_step(k, y, v, w.x, w.y);
+        heston_px_step(k, y, v, w.z, w.w);
+    }
+    if (steps & 1) {
+        const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
+        heston_px_step(k, y, v, w.x, w.y);
+    }
+    return y * k.ck;  // log2(S_T / spot)
 }
 
 // ----------------------------------------------------------------------------
@@ -358,11 +431,26 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
             k.rc = R(rec.c[7]);
             k.xf = R(log2(1e-12 / rec.spot));
             k.v0 = R(rec.v0);
+            HestonPx kp;
+            if constexpr (MODE == SIM_PHILOX_F32) {
+                const double sq = 1.1774100225154746910;  // sqrt(2 ln 2)
+                const double ck = rec.c[2] * sq;          // sqrt(dt) log2e sqrt(2 ln 2)
+                kp.a = float(rec.c[0] / ck);
+                kp.h = float(rec.c[1] / ck);
+                kp.omk = float(1.0 - rec.c[3]);
+                kp.kth = float(rec.c[4]);
+                kp.sgk = float(rec.c[5] * sq);
+                kp.phi = float(atan2(rec.c[7], rec.c[6]) * 0.15915494309189533577);  // phi / 2 pi
+                kp.yf = float(log2(1e-12 / rec.spot) / ck);
+                kp.v0 = float(rec.v0);
+                kp.ck = float(ck);
+            }
             for (int32_t j = 0; j < lane_paths; ++j) {
                 const int64_t p = first + threadIdx.x + int64_t(j) * kBlock;
                 if (p >= paths) break;
                 R x;
-                if constexpr (MODE == SIM_PHILOX_F32) x = heston_path<R>(ph, uint32_t(p), steps, k);
+                if constexpr (MODE == SIM_PHILOX_F32)
+                    x = heston_path_px(uint32_t(p), steps, kp, rec.seed_lo, rec.seed_hi);
                 else x = heston_path<R>(ss, uint32_t(p), steps, k);
                 const double f = double(payoff(spot * q_exp2(x), strike, ot)) - shift;
                 s1 += f;
@@ -541,10 +629,15 @@ hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_de
     return hipGetLastError();
 }
 
-hipError_t launch_scan(hipStream_t st, const int32_t* n_dev, const int32_t* ntiles,
-                       OptionRec* recs, int32_t* tile_start, int32_t* counters) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, n_dev, ntiles, recs, tile_start,
-                       counters);
+hipError_t launch_scan(hipStream_t st, int32_t n, const int32_t* n_dev, const int32_t* ntiles,
+                       OptionRec* recs, int32_t* tile_start, int32_t* counters, void* scratch) {
+    // n + 1 elements so that the thread of index n writes tile_start[n]
+    const int32_t nb = (n + 1 + kScanBlock - 1) / kScanBlock;
+    long long* bs = static_cast<long long*>(scratch);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, st, n_dev, ntiles, bs);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, st, nb, bs, counters);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, st, n_dev, ntiles, bs, counters,
+                       recs, tile_start);
     return hipGetLastError();
 }
 

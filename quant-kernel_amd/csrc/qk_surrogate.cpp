@@ -515,7 +515,7 @@ int32_t sur_heston_price(const BatchIn& in_arg, int32_t n, bool device, double* 
         BatchIn sp = in;
         sp.active = P<uint8_t>(g_sur.act_spot);
         sp.paths_override = g_sur.spot_paths;
-        rc = mc_device_locked(d, sp, n_dev, P<double>(g_sur.spot_p), P<double>(g_sur.spot_se), err);
+        rc = mc_device_locked(d, sp, n, n_dev, P<double>(g_sur.spot_p), P<double>(g_sur.spot_se), err);
         if (rc != QK_OK) return rc;
         QK_HIP(sur_compare(st, in, n, g_sur.spot_k, P<uint8_t>(g_sur.act_spot),
                            P<double>(g_sur.sur), P<double>(g_sur.spot_p),
@@ -525,7 +525,7 @@ int32_t sur_heston_price(const BatchIn& in_arg, int32_t n, bool device, double* 
     // 4. fallback MC for out-of-box queries and failed spot checks
     BatchIn fb = in;
     fb.active = P<uint8_t>(g_sur.act_fb);
-    rc = mc_device_locked(d, fb, n_dev, P<double>(g_sur.fb_p), P<double>(g_sur.fb_se), err);
+    rc = mc_device_locked(d, fb, n, n_dev, P<double>(g_sur.fb_p), P<double>(g_sur.fb_se), err);
     if (rc != QK_OK) return rc;
     QK_HIP(sur_combine(st, n, P<int8_t>(g_sur.status), P<double>(g_sur.sur), P<double>(g_sur.fb_p),
                        P<double>(g_sur.closed), dprice, dsrc));
