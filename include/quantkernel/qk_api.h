@@ -210,6 +210,30 @@ QK_EXPORT int32_t qk_mc_validation_stream(uint64_t seed, int64_t count, double* 
 QK_EXPORT int32_t qk_mc_philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3,
                                      int32_t count, uint32_t* out_words, float* out_normals);
 
+/* Launch graphs of the host-memory QK_RNG_PHILOX path (on by default): each
+ * distinct batch shape is captured once (H2D copy, prepare, scan, simulate,
+ * finalize, D2H copy) and replayed.  enable: 1 on, 0 off, -1 unchanged.
+ * out_cached: graphs held (-1 if the runtime cannot capture); out_launches:
+ * graph launches so far.  Either output may be null.  QK_NO_GRAPH=1 turns
+ * graphs off for the process. */
+QK_EXPORT int32_t qk_mc_graphs(int32_t enable, int32_t* out_cached, int64_t* out_launches);
+
+/* Tile shapes of the persistent simulation kernels.  family 1 = terminal
+ * (lane_units = Philox groups of 4 antithetic pairs per lane), 2 = stepwise
+ * (lane_units = paths per lane); grid_per_cu = persistent workgroups per CU.
+ * Results are bitwise reproducible for a fixed shape; another shape changes
+ * the summation order (differences ~1e-16 relative).  Defaults: 8 and 8. */
+QK_EXPORT int32_t qk_mc_set_shape(int32_t family, int32_t lane_units, int32_t grid_per_cu);
+QK_EXPORT int32_t qk_mc_get_shape(int32_t family, int32_t* lane_units, int32_t* grid_per_cu);
+/* Autotuner: times the candidate shapes of a family (lane_units in
+ * {2,4,8,16,32} x grid_per_cu in {4,8,16}) on a representative synthetic batch
+ * (C2-like uneven stepwise mix, or 4096 x 100k terminal), applies the fastest
+ * and appends it to $QK_AUTOTUNE_CACHE (default ~/.cache/
+ * quantkernel_mi355x_autotune.txt); with QK_AUTOTUNE=1 the cached shapes are
+ * applied when the device is opened. */
+QK_EXPORT int32_t qk_mc_autotune(int32_t family, int32_t* out_lane_units, int32_t* out_grid_per_cu,
+                                 double* out_ms);
+
 /* Name of the device in use and its compute-unit count; QK_ERR_RUNTIME_INIT
  * if there is none. */
 QK_EXPORT int32_t qk_device_info(char* name_buf, int32_t buf_len, int32_t* compute_units);

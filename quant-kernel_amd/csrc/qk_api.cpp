@@ -404,6 +404,36 @@ int32_t qk_mc_philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3,
     return rc;
 }
 
+int32_t qk_mc_graphs(int32_t enable, int32_t* out_cached, int64_t* out_launches) {
+    std::string err;
+    const int32_t rc = qk::set_graphs(enable, out_cached, out_launches, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
+int32_t qk_mc_set_shape(int32_t family, int32_t lane_units, int32_t grid_per_cu) {
+    std::string err;
+    const int32_t rc = qk::set_shape(family, lane_units, grid_per_cu, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
+int32_t qk_mc_get_shape(int32_t family, int32_t* lane_units, int32_t* grid_per_cu) {
+    if (!check_ptrs({QK_P(lane_units), QK_P(grid_per_cu)})) return QK_ERR_NULL_PTR;
+    std::string err;
+    const int32_t rc = qk::get_shape(family, lane_units, grid_per_cu, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
+int32_t qk_mc_autotune(int32_t family, int32_t* out_lane_units, int32_t* out_grid_per_cu,
+                       double* out_ms) {
+    std::string err;
+    const int32_t rc = qk::autotune(family, out_lane_units, out_grid_per_cu, out_ms, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
 int32_t qk_mc_validation_stream(uint64_t seed, int64_t count, double* out) {
     if (!check_ptrs({QK_P(out)})) return QK_ERR_NULL_PTR;
     if (count < 0) {

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "qk_kernels.hpp"
 
@@ -95,6 +96,17 @@ struct Device {
 
     SimShape shape_step{8, 0};
     SimShape shape_term{8, 0};
+
+    // instantiated launch graphs of the host-memory MC path, keyed by
+    // everything their nodes depend on (sizes, shapes and buffer addresses)
+    struct Graph {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
+    bool graphs_ok = true;        // capture and instantiation work on this runtime
+    bool graphs_enabled = true;   // caller switch (qk_mc_set_graphs)
+    int64_t graph_launches = 0;   // replays + first launches, for diagnostics
 };
 
 // Returns the context (creating it on first use) or null with *err set.

@@ -17,6 +17,8 @@
 #include <cstring>
 #include <mutex>
 #include <random>
+#include <fstream>
+#include <sstream>
 #include <thread>
 #include <vector>
 
@@ -27,6 +29,9 @@ namespace {
 std::atomic<int32_t> g_rng_mode{QK_RNG_PHILOX};
 
 Device* g_dev = nullptr;
+}  // namespace
+void apply_cached_shapes(Device* d);
+namespace {
 std::string g_init_err;
 std::once_flag g_once;
 
@@ -76,6 +81,7 @@ void init_device() {
     d->shape_step.grid = d->cus * per_cu;
     d->shape_term.lane_units = std::max(1, env_int("QK_LANE_GROUPS", 8));
     d->shape_term.grid = d->cus * per_cu;
+    if (env_int("QK_AUTOTUNE", 0)) apply_cached_shapes(d);
     g_dev = d;
 }
 
@@ -235,7 +241,7 @@ static void fill_streams(const HostBatch& hb, int32_t n, const std::vector<Optio
 // Packs a host SoA batch (stride align64(n)) into pinned memory and queues
 // one H2D copy into the device staging buffer; fills *bi and *n_dev.
 int32_t stage_host_batch(Device* d, const HostBatch& hb, int32_t n, BatchIn* bi_out,
-                         const int32_t** n_dev_out, std::string* err) {
+                         const int32_t** n_dev_out, std::string* err, size_t* bytes_out) {
     const size_t ns = align64(size_t(n));
     const double* dsrc[11] = {hb.spot, hb.strike, hb.t, hb.vol, hb.r, hb.q,
                               hb.v0, hb.kappa, hb.theta, hb.sigma, hb.rho};
@@ -276,11 +282,66 @@ int32_t stage_host_batch(Device* d, const HostBatch& hb, int32_t n, BatchIn* bi_
     off += 64;
     bi.uniform_method = hb.uniform_method;
     bi.method_is_model = hb.model_ids ? 1 : 0;
-    QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, off, hipMemcpyHostToDevice, d->stream));
+    if (!bytes_out)
+        QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, off, hipMemcpyHostToDevice, d->stream));
+    else
+        *bytes_out = off;  // the caller enqueues the copy (inside its graph)
     *bi_out = bi;
     *n_dev_out = n_dev;
     return QK_OK;
 }
+
+namespace {
+
+template <class F>
+int32_t enqueue_captured(F& enqueue, std::string*) {
+    return enqueue();
+}
+
+// Runs `enqueue` on d->stream through a cached instantiated graph when
+// possible.  The first call for a key captures and instantiates; later calls
+// with the same key only launch.  Any capture or instantiation failure turns
+// graphs off for the process and runs the work directly.
+template <class F>
+int32_t run_graphed(Device* d, const std::vector<uint64_t>& key, F&& enqueue, std::string* err) {
+    static const bool disabled = env_int("QK_NO_GRAPH", 0) != 0;
+    if (disabled || !d->graphs_ok || !d->graphs_enabled) return enqueue();
+    for (auto& g : d->graphs)
+        if (g.key == key) {
+            QK_HIP(hipGraphLaunch(g.exec, d->stream));
+            ++d->graph_launches;
+            return QK_OK;
+        }
+    hipError_t e = hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        d->graphs_ok = false;
+        return enqueue();
+    }
+    std::string cap_err;
+    const int32_t rc = enqueue_captured(enqueue, &cap_err);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(d->stream, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (rc == QK_OK && e == hipSuccess && graph)
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != QK_OK || e != hipSuccess || !exec) {
+        (void)hipGetLastError();
+        d->graphs_ok = false;
+        return enqueue();
+    }
+    if (d->graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(d->graphs.front().exec);
+        d->graphs.erase(d->graphs.begin());
+    }
+    d->graphs.push_back(Device::Graph{key, exec});
+    QK_HIP(hipGraphLaunch(exec, d->stream));
+    ++d->graph_launches;
+    return QK_OK;
+}
+
+}  // namespace
+
 
 int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* out_se,
                      std::string* err) {
@@ -303,7 +364,10 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
     const size_t ns = align64(size_t(n));
     BatchIn bi{};
     const int32_t* n_dev = nullptr;
-    int32_t prc = stage_host_batch(d, hb, n, &bi, &n_dev, err);
+    // Philox batches run as one cached graph: H2D, kernels, D2H
+    const bool graphed = mode == QK_RNG_PHILOX;
+    size_t in_copy = 0;
+    int32_t prc = stage_host_batch(d, hb, n, &bi, &n_dev, err, graphed ? &in_copy : nullptr);
     if (prc != QK_OK) return prc;
 
     // ---- device work buffers ----
@@ -376,12 +440,41 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
         d_stream = d->stream_buf.p;
     }
 
-    int32_t rc = launch_pipeline(d, bi, n, n_dev, d_off, family, sim_mode, d_stream, dprice, dse, err);
+    auto enqueue = [&]() -> int32_t {
+        if (graphed)
+            QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, in_copy, hipMemcpyHostToDevice, d->stream));
+        int32_t rc = launch_pipeline(d, bi, n, n_dev, d_off, family, sim_mode, d_stream, dprice, dse,
+                                     err);
+        if (rc != QK_OK) return rc;
+        // outputs + the scan's overflow flag
+        QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, ns * 16, hipMemcpyDeviceToHost, d->stream));
+        QK_HIP(hipMemcpyAsync(static_cast<char*>(d->pin_out.p) + ns * 16, d->counters.p, 16,
+                              hipMemcpyDeviceToHost, d->stream));
+        return QK_OK;
+    };
+    int32_t rc;
+    if (graphed) {
+        const SimShape& st = d->shape_step;
+        const SimShape& tt = d->shape_term;
+        uint64_t present = 0;
+        const void* fields[] = {bi.method, bi.spot, bi.strike, bi.t, bi.vol, bi.r, bi.q, bi.v0,
+                                bi.kappa, bi.theta, bi.sigma, bi.rho, bi.ot, bi.paths, bi.steps,
+                                bi.seed};
+        for (int k = 0; k < 16; ++k) present |= uint64_t(fields[k] != nullptr) << k;
+        const std::vector<uint64_t> key = {
+            uint64_t(n), uint64_t(family), uint64_t(bi.uniform_method), uint64_t(bi.method_is_model),
+            present, uint64_t(st.lane_units), uint64_t(st.grid), uint64_t(tt.lane_units),
+            uint64_t(tt.grid), uint64_t(in_copy), uint64_t(uintptr_t(d->in.p)),
+            uint64_t(uintptr_t(d->pin_in.p)), uint64_t(uintptr_t(d->recs.p)),
+            uint64_t(uintptr_t(d->ntiles.p)), uint64_t(uintptr_t(d->tile_start.p)),
+            uint64_t(uintptr_t(d->counters.p)), uint64_t(uintptr_t(d->scan.p)),
+            uint64_t(uintptr_t(d->partials.p)), uint64_t(uintptr_t(d->out.p)),
+            uint64_t(uintptr_t(d->pin_out.p))};
+        rc = run_graphed(d, key, enqueue, err);
+    } else {
+        rc = enqueue();
+    }
     if (rc != QK_OK) return rc;
-    // outputs + the scan's overflow flag
-    QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, ns * 16, hipMemcpyDeviceToHost, d->stream));
-    QK_HIP(hipMemcpyAsync(static_cast<char*>(d->pin_out.p) + ns * 16, d->counters.p, 16,
-                          hipMemcpyDeviceToHost, d->stream));
     QK_HIP(hipStreamSynchronize(d->stream));
     record_timing(d);
     const int32_t* hc = reinterpret_cast<const int32_t*>(static_cast<char*>(d->pin_out.p) + ns * 16);
@@ -602,6 +695,221 @@ int32_t run_lsmc_batch(const double* spot, const double* strike, const double* t
         if (out_se) out_se[rc.out_index] = hp[ns + rc.out_index];
     }
     return QK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Shapes and the autotuner
+// ---------------------------------------------------------------------------
+namespace {
+
+std::string autotune_path() {
+    const char* p = std::getenv("QK_AUTOTUNE_CACHE");
+    if (p && *p) return p;
+    const char* home = std::getenv("HOME");
+    return std::string(home && *home ? home : "/tmp") + "/.cache/quantkernel_mi355x_autotune.txt";
+}
+
+SimShape* shape_of(Device* d, int32_t family) {
+    return family == FAM_TERMINAL ? &d->shape_term : family == FAM_STEPWISE ? &d->shape_step : nullptr;
+}
+
+// Synthetic batch of one family in device memory (freed by the caller).
+struct TuneBatch {
+    std::vector<void*> bufs;
+    BatchIn in{};
+    int32_t n = 0;
+    double *price = nullptr, *se = nullptr;
+    int32_t* n_dev = nullptr;
+};
+
+template <typename T>
+T* upload(TuneBatch& tb, const std::vector<T>& v, std::string* err, int32_t* rc) {
+    void* p = nullptr;
+    if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess ||
+        hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+        if (err) *err = "autotune: device allocation failed";
+        *rc = QK_ERR_RUNTIME_INIT;
+        return nullptr;
+    }
+    tb.bufs.push_back(p);
+    return static_cast<T*>(p);
+}
+
+int32_t make_tune_batch(int32_t family, TuneBatch& tb, std::string* err) {
+    const int32_t n = 4096;
+    std::vector<double> spot(n), strike(n), t(n), vol(n), r(n), q(n), v0(n), ka(n), th(n), sg(n), rho(n);
+    std::vector<int32_t> model(n), ot(n), paths(n), steps(n);
+    std::vector<uint64_t> seed(n);
+    for (int32_t i = 0; i < n; ++i) {
+        spot[i] = 100.0;
+        strike[i] = 80.0 + 40.0 * (i % 97) / 96.0;
+        t[i] = 0.25 + 1.5 * (i % 13) / 12.0;
+        vol[i] = 0.15 + 0.3 * (i % 7) / 6.0;
+        r[i] = 0.03;
+        q[i] = 0.0;
+        v0[i] = th[i] = 0.04;
+        ka[i] = 2.0;
+        sg[i] = 0.5;
+        rho[i] = -0.6;
+        model[i] = i % 4;
+        ot[i] = i & 1;
+        // C2-like imbalance for the stepwise family, C0 for the terminal one
+        paths[i] = family == FAM_TERMINAL ? 100000 : (4096 << (i % 5));
+        steps[i] = family == FAM_TERMINAL ? 1 : (16 << (i % 4));
+        seed[i] = 1234567u + uint64_t(i);
+    }
+    int32_t rc = QK_OK;
+    tb.n = n;
+    tb.in.spot = upload(tb, spot, err, &rc);
+    tb.in.strike = upload(tb, strike, err, &rc);
+    tb.in.t = upload(tb, t, err, &rc);
+    tb.in.vol = upload(tb, vol, err, &rc);
+    tb.in.r = upload(tb, r, err, &rc);
+    tb.in.q = upload(tb, q, err, &rc);
+    tb.in.v0 = upload(tb, v0, err, &rc);
+    tb.in.kappa = upload(tb, ka, err, &rc);
+    tb.in.theta = upload(tb, th, err, &rc);
+    tb.in.sigma = upload(tb, sg, err, &rc);
+    tb.in.rho = upload(tb, rho, err, &rc);
+    tb.in.ot = upload(tb, ot, err, &rc);
+    tb.in.paths = upload(tb, paths, err, &rc);
+    tb.in.steps = upload(tb, steps, err, &rc);
+    tb.in.seed = upload(tb, seed, err, &rc);
+    if (family == FAM_TERMINAL) {
+        tb.in.uniform_method = M_STANDARD;
+    } else {
+        tb.in.method = upload(tb, model, err, &rc);
+        tb.in.method_is_model = 1;
+    }
+    tb.n_dev = upload(tb, std::vector<int32_t>{n}, err, &rc);
+    tb.price = upload(tb, std::vector<double>(static_cast<size_t>(n)), err, &rc);
+    tb.se = upload(tb, std::vector<double>(static_cast<size_t>(n)), err, &rc);
+    return rc;
+}
+
+void free_tune_batch(TuneBatch& tb) {
+    for (void* p : tb.bufs) (void)hipFree(p);
+    tb.bufs.clear();
+}
+
+}  // namespace
+
+int32_t set_graphs(int32_t enable, int32_t* cached, int64_t* launches, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    if (enable >= 0) d->graphs_enabled = enable != 0;
+    if (cached) *cached = d->graphs_ok ? int32_t(d->graphs.size()) : -1;
+    if (launches) *launches = d->graph_launches;
+    return QK_OK;
+}
+
+int32_t set_shape(int32_t family, int32_t lane_units, int32_t grid_per_cu, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    SimShape* sh = shape_of(d, family);
+    if (!sh || lane_units < 1 || lane_units > 64 || grid_per_cu < 1 || grid_per_cu > 64) {
+        if (err) *err = "shape: family 1 (terminal) or 2 (stepwise), 1 <= lane_units, grid_per_cu <= 64";
+        return QK_ERR_INVALID_INPUT;
+    }
+    sh->lane_units = lane_units;
+    sh->grid = d->cus * grid_per_cu;
+    return QK_OK;
+}
+
+int32_t get_shape(int32_t family, int32_t* lane_units, int32_t* grid_per_cu, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    SimShape* sh = shape_of(d, family);
+    if (!sh) {
+        if (err) *err = "shape: family 1 (terminal) or 2 (stepwise)";
+        return QK_ERR_INVALID_INPUT;
+    }
+    if (lane_units) *lane_units = sh->lane_units;
+    if (grid_per_cu) *grid_per_cu = sh->grid / std::max(1, d->cus);
+    return QK_OK;
+}
+
+// Times every candidate shape of a family on a representative synthetic batch
+// (the simulate kernel alone, best of two runs), applies the fastest and
+// appends it to the cache file.
+int32_t autotune(int32_t family, int32_t* best_lane, int32_t* best_grid, double* best_ms,
+                 std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+    SimShape* sh = shape_of(d, family);
+    if (!sh) {
+        if (err) *err = "autotune: family 1 (terminal) or 2 (stepwise)";
+        return QK_ERR_INVALID_INPUT;
+    }
+    TuneBatch tb;
+    int32_t rc = make_tune_batch(family, tb, err);
+    if (rc == QK_OK) rc = ensure_mc_capacity(d, tb.n, err);
+    const SimShape saved = *sh;
+    SimShape best = saved;
+    double best_t = 1e300;
+    const int lanes[] = {2, 4, 8, 16, 32};
+    const int grids[] = {4, 8, 16};
+    for (int li = 0; rc == QK_OK && li < 5; ++li)
+        for (int gi = 0; rc == QK_OK && gi < 3; ++gi) {
+            sh->lane_units = lanes[li];
+            sh->grid = d->cus * grids[gi];
+            double t = 1e300;
+            for (int rep = 0; rep < 2 && rc == QK_OK; ++rep) {
+                rc = mc_device_locked(d, tb.in, tb.n, tb.n_dev, tb.price, tb.se, err);
+                if (rc != QK_OK) break;
+                if (hipStreamSynchronize(d->stream) != hipSuccess) {
+                    rc = QK_ERR_RUNTIME_INIT;
+                    break;
+                }
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, d->ev[1], d->ev[2]) == hipSuccess)
+                    t = std::min(t, double(ms));
+            }
+            if (t < best_t) {
+                best_t = t;
+                best = *sh;
+            }
+        }
+    free_tune_batch(tb);
+    if (rc != QK_OK) {
+        *sh = saved;
+        return rc;
+    }
+    *sh = best;
+    if (best_lane) *best_lane = best.lane_units;
+    if (best_grid) *best_grid = best.grid / std::max(1, d->cus);
+    if (best_ms) *best_ms = best_t;
+    std::ofstream f(autotune_path(), std::ios::app);
+    if (f)
+        f << d->name << "|" << d->cus << "|" << family << "|" << best.lane_units << "|"
+          << best.grid / std::max(1, d->cus) << "|" << best_t << "\n";
+    return QK_OK;
+}
+
+// Applies the last cached result for this device, if any (QK_AUTOTUNE=1).
+void apply_cached_shapes(Device* d) {
+    std::ifstream f(autotune_path());
+    std::string line;
+    while (std::getline(f, line)) {
+        std::stringstream ss(line);
+        std::string name, cus, fam, lane, grid;
+        if (!std::getline(ss, name, '|') || !std::getline(ss, cus, '|') ||
+            !std::getline(ss, fam, '|') || !std::getline(ss, lane, '|') ||
+            !std::getline(ss, grid, '|'))
+            continue;
+        if (name != d->name || std::atoi(cus.c_str()) != d->cus) continue;
+        SimShape* sh = shape_of(d, std::atoi(fam.c_str()));
+        const int lu = std::atoi(lane.c_str()), g = std::atoi(grid.c_str());
+        if (sh && lu >= 1 && lu <= 64 && g >= 1 && g <= 64) {
+            sh->lane_units = lu;
+            sh->grid = d->cus * g;
+        }
+    }
 }
 
 }  // namespace qk

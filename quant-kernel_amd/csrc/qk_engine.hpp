@@ -40,8 +40,10 @@ int32_t mc_device_locked(Device* d, const BatchIn& in, int32_t n, const int32_t*
                          double* out_price_dev, double* out_se_dev, std::string* err);
 int32_t ensure_mc_capacity(Device* d, int32_t n, std::string* err);
 // Packs a host batch into the device staging buffer (one H2D copy, queued).
+// With bytes_out set, the copy is not queued and its size is returned
+// (d->pin_in -> d->in), for callers that capture it in a graph.
 int32_t stage_host_batch(Device* d, const HostBatch& hb, int32_t n, BatchIn* bi,
-                         const int32_t** n_dev, std::string* err);
+                         const int32_t** n_dev, std::string* err, size_t* bytes_out = nullptr);
 
 // Process-wide RNG mode (QK_RNG_*).
 void set_rng_mode(int32_t mode);
@@ -54,6 +56,17 @@ void host_normal_stream(uint64_t seed, int64_t count, double* out);
 // Device Philox words and FP32 normals of blocks (c0+i, c1, 0, c3), i < count.
 int32_t philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3, int32_t count,
                      uint32_t* words, float* normals, std::string* err);
+
+// Tile shape of a family (FAM_TERMINAL / FAM_STEPWISE): paths (stepwise) or
+// Philox groups (terminal) per lane, and persistent workgroups per CU.
+int32_t set_shape(int32_t family, int32_t lane_units, int32_t grid_per_cu, std::string* err);
+int32_t get_shape(int32_t family, int32_t* lane_units, int32_t* grid_per_cu, std::string* err);
+int32_t autotune(int32_t family, int32_t* best_lane, int32_t* best_grid, double* best_ms,
+                 std::string* err);
+
+// Graph switch (enable < 0 leaves it) and counters: cached graphs (-1 when
+// capture is unsupported) and graph launches so far.
+int32_t set_graphs(int32_t enable, int32_t* cached, int64_t* launches, std::string* err);
 
 // Event timings of the last completed batch.
 bool last_timing(double* ms_total, double* ms_simulate);

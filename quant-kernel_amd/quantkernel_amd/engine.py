@@ -115,6 +115,10 @@ _SIGNATURES = {
     "qk_device_info": (_I, [ct.c_char_p, _I, _PI]),
     "qk_mc_philox_probe": (_I, [_U, ct.c_uint32, ct.c_uint32, ct.c_uint32, _I,
                                 ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_float)]),
+    "qk_mc_graphs": (_I, [_I, _PI, ct.POINTER(ct.c_int64)]),
+    "qk_mc_set_shape": (_I, [_I, _I, _I]),
+    "qk_mc_get_shape": (_I, [_I, _PI, _PI]),
+    "qk_mc_autotune": (_I, [_I, _PI, _PI, _PD]),
     "qk_sur_param_count": (_I, []),
     "qk_sur_init": (_I, [_U]),
     "qk_sur_train": (_I, [_PD, _PD, _I, _I, _I, _D, _D, _U, _PD]),
@@ -204,6 +208,27 @@ class QuantKernel:
         cus = ct.c_int32()
         self._check(self._lib.qk_device_info(buf, 256, ct.byref(cus)), "qk_device_info")
         return buf.value.decode(), int(cus.value)
+
+    def graphs(self, enable: int = -1) -> tuple[int, int]:
+        c, n = ct.c_int32(), ct.c_int64()
+        self._check(self._lib.qk_mc_graphs(int(enable), ct.byref(c), ct.byref(n)), "qk_mc_graphs")
+        return c.value, n.value
+
+    FAMILY_TERMINAL = 1
+    FAMILY_STEPWISE = 2
+
+    def set_shape(self, family: int, lane_units: int, grid_per_cu: int) -> None:
+        self._check(self._lib.qk_mc_set_shape(int(family), int(lane_units), int(grid_per_cu)), "qk_mc_set_shape")
+
+    def get_shape(self, family: int) -> tuple[int, int]:
+        a, b = ct.c_int32(), ct.c_int32()
+        self._check(self._lib.qk_mc_get_shape(int(family), ct.byref(a), ct.byref(b)), "qk_mc_get_shape")
+        return a.value, b.value
+
+    def autotune(self, family: int) -> tuple[int, int, float]:
+        a, b, t = ct.c_int32(), ct.c_int32(), ct.c_double()
+        self._check(self._lib.qk_mc_autotune(int(family), ct.byref(a), ct.byref(b), ct.byref(t)), "qk_mc_autotune")
+        return a.value, b.value, t.value
 
     def validation_stream(self, seed: int, count: int) -> np.ndarray:
         out = np.empty(int(count), dtype=np.float64)

@@ -297,3 +297,48 @@ def test_device_philox_bitwise_and_normals(qk, orc):
                 orc.lib.orc_box_muller(int(words[i][j]), int(words[i][j + 1]), ct.byref(a), ct.byref(b))
                 assert abs(normals[i][j] - a.value) < 2e-6 * max(1.0, abs(a.value))
                 assert abs(normals[i][j + 1] - b.value) < 2e-6 * max(1.0, abs(b.value))
+
+
+def test_graph_replays_match_direct_launches(qk):
+    """The host path's captured graph, replayed with other device work in
+    between, gives bitwise the results of direct launches."""
+    from quantkernel_amd import workloads as wl
+    qk.graphs(1)
+    d = wl.config1(40, paths=3000, steps=24)
+    keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
+    args = [d[k] for k in keys] + [d["option_type"], d["paths"], d["steps"], d["seed"]]
+    runs = []
+    for i in range(4):
+        runs.append(qk.heston_monte_carlo_price_batch(*args, return_stderr=True))
+        # unrelated eager work between replays: another shape, another family
+        e = wl.config0(17, paths=999)
+        qk.standard_monte_carlo_price_batch(*[e[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type",
+                                                             "paths", "seed")])
+    cached, launches = qk.graphs(0)
+    direct = qk.heston_monte_carlo_price_batch(*args, return_stderr=True)
+    qk.graphs(1)
+    print("graphs cached", cached, "launches", launches)
+    assert cached != -1 and launches >= 4
+    for p, s in runs:
+        assert np.array_equal(p, direct[0]) and np.array_equal(s, direct[1])
+
+
+def test_autotune_and_shapes(qk):
+    from quantkernel_amd import workloads as wl
+    fam = qk.FAMILY_STEPWISE
+    saved = qk.get_shape(fam)
+    d = wl.config2(60, p_lo=2000, p_hi=30000, s_lo=4, s_hi=40)
+    a = [d[k] for k in ("model", "spot", "strike", "t", "vol", "r", "q", "v0", "kappa", "theta", "sigma", "rho",
+                        "option_type", "paths", "steps", "seed")]
+    base, _ = qk.mixed_price_batch(*a)
+    lane, grid, ms = qk.autotune(fam)
+    print("autotuned stepwise shape", lane, grid, ms)
+    assert qk.get_shape(fam) == (lane, grid) and ms > 0
+    tuned, _ = qk.mixed_price_batch(*a)
+    # another tile shape only changes the summation order
+    assert np.allclose(tuned, base, rtol=1e-12, atol=1e-12)
+    qk.set_shape(fam, 3, 5)
+    odd, _ = qk.mixed_price_batch(*a)
+    assert np.allclose(odd, base, rtol=1e-12, atol=1e-12)
+    qk.set_shape(fam, *saved)
+    assert np.array_equal(qk.mixed_price_batch(*a)[0], base)
