@@ -126,6 +126,59 @@ QK_EXPORT int32_t qk_mcm_local_vol_monte_carlo_price_batch(const double* spot, c
                                                            const uint64_t* seed,
                                                            int32_t n, double* out_prices);
 
+/* ---- importance sampling (reference qk_api.h:288-290, 335-340) ---------- */
+QK_EXPORT double  qk_mcm_importance_sampling_price(double spot, double strike, double t, double vol,
+                                                   double r, double q, int32_t option_type,
+                                                   int32_t paths, double shift, uint64_t seed);
+QK_EXPORT int32_t qk_mcm_importance_sampling_price_batch(const double* spot, const double* strike,
+                                                         const double* t, const double* vol,
+                                                         const double* r, const double* q,
+                                                         const int32_t* option_type, const int32_t* paths,
+                                                         const double* shift, const uint64_t* seed,
+                                                         int32_t n, double* out_prices);
+
+/* ---- Heston likelihood-ratio delta (reference qk_api.h:375-392) --------- */
+QK_EXPORT double  qk_mcm_heston_lr_delta(double spot, double strike, double t,
+                                         double r, double q,
+                                         double v0, double kappa, double theta,
+                                         double sigma, double rho,
+                                         int32_t option_type,
+                                         int32_t paths, int32_t steps, uint64_t seed,
+                                         double weight_clip);
+QK_EXPORT int32_t qk_mcm_heston_lr_delta_batch(const double* spot, const double* strike,
+                                               const double* t,
+                                               const double* r, const double* q,
+                                               const double* v0, const double* kappa,
+                                               const double* theta, const double* sigma,
+                                               const double* rho,
+                                               const int32_t* option_type,
+                                               const int32_t* paths, const int32_t* steps,
+                                               const uint64_t* seed,
+                                               const double* weight_clip,
+                                               int32_t n, double* out_prices);
+
+/* ---- pathwise and likelihood-ratio delta (reference qk_api.h:609-614,
+ *      618-630) ------------------------------------------------------------ */
+QK_EXPORT double  qk_agm_pathwise_derivative_delta(double spot, double strike, double t, double vol,
+                                                   double r, double q, int32_t option_type,
+                                                   int32_t paths, uint64_t seed);
+QK_EXPORT double  qk_agm_likelihood_ratio_delta(double spot, double strike, double t, double vol,
+                                                double r, double q, int32_t option_type,
+                                                int32_t paths, uint64_t seed, double weight_clip);
+QK_EXPORT int32_t qk_agm_pathwise_derivative_delta_batch(const double* spot, const double* strike,
+                                                         const double* t, const double* vol,
+                                                         const double* r, const double* q,
+                                                         const int32_t* option_type,
+                                                         const int32_t* paths, const uint64_t* seed,
+                                                         int32_t n, double* out_prices);
+QK_EXPORT int32_t qk_agm_likelihood_ratio_delta_batch(const double* spot, const double* strike,
+                                                      const double* t, const double* vol,
+                                                      const double* r, const double* q,
+                                                      const int32_t* option_type,
+                                                      const int32_t* paths, const uint64_t* seed,
+                                                      const double* weight_clip,
+                                                      int32_t n, double* out_prices);
+
 /* ======================================================================== *
  * ABI 2.13 additions.  No reference counterpart: new names only.
  * ======================================================================== */
@@ -161,6 +214,30 @@ QK_EXPORT int32_t qk_mcm_heston_monte_carlo_price_batch_se(const double* spot, c
         const double* v0, const double* kappa, const double* theta, const double* sigma,
         const double* rho, const int32_t* option_type, const int32_t* paths, const int32_t* steps,
         const uint64_t* seed, int32_t n, double* out_prices, double* out_stderr);
+
+/* Standard errors of importance sampling (per path), of the pathwise delta
+ * (per path), of the likelihood-ratio delta (per antithetic pair) and of the
+ * Heston likelihood-ratio delta (of sum / paths, with the antithetic pair as
+ * the sample unit).  Same argument order as the batch entry points above plus
+ * out_stderr. */
+QK_EXPORT int32_t qk_mcm_importance_sampling_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const double* shift, const uint64_t* seed,
+        int32_t n, double* out_prices, double* out_stderr);
+QK_EXPORT int32_t qk_mcm_heston_lr_delta_batch_se(const double* spot, const double* strike,
+        const double* t, const double* r, const double* q,
+        const double* v0, const double* kappa, const double* theta, const double* sigma,
+        const double* rho, const int32_t* option_type, const int32_t* paths, const int32_t* steps,
+        const uint64_t* seed, const double* weight_clip, int32_t n, double* out_prices,
+        double* out_stderr);
+QK_EXPORT int32_t qk_agm_pathwise_derivative_delta_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const uint64_t* seed,
+        int32_t n, double* out_prices, double* out_stderr);
+QK_EXPORT int32_t qk_agm_likelihood_ratio_delta_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const uint64_t* seed,
+        const double* weight_clip, int32_t n, double* out_prices, double* out_stderr);
 
 /* Mixed-model uneven batch: one persistent launch over options of different
  * models (QK_MODEL_*), path counts and step counts.  vol is used by the GBM

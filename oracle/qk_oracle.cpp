@@ -744,3 +744,295 @@ ORC_EXPORT void orc_standard_mc_batch(const double* spot, const double* strike, 
         out[i] = orc_standard_mc(spot[i], strike[i], t[i], vol[i], r[i], q[i], ot[i], paths[i],
                                  seed[i]);
 }
+
+// ===========================================================================
+// 4. Callers of estimate_terminal (mc_engine.h:62-75) and the Greeks on the
+//    MC engine: importance sampling, pathwise and likelihood-ratio delta
+//    (GBM), Heston likelihood-ratio delta.  Each exists twice: on the
+//    reference's mt19937_64 stream (bit-identical to the reference when both
+//    are built the same way) and on the GPU's Philox stream with the
+//    per-estimator-unit standard error.
+// ===========================================================================
+namespace {
+
+// adjoint_greeks/common/internal_util.h:22-31
+double deterministic_delta(double spot, double strike, double t, double r, double q, int32_t ot) {
+    const double forward = spot * std::exp((r - q) * t);
+    const double qf = std::exp(-q * t);
+    if (std::fabs(forward - strike) < kEps * strike) return ot == 0 ? 0.5 * qf : -0.5 * qf;
+    if (ot == 0) return forward > strike ? qf : 0.0;
+    return forward < strike ? -qf : 0.0;
+}
+
+// adjoint_greeks/common/internal_util.h:47-50
+double clamp_delta(double delta, double t, double q) {
+    const double qf = std::exp(-q * t);
+    return std::min(qf, std::max(-qf, delta));
+}
+
+// importance_sampling.cpp:26-31: the accumulated value of one path.
+double is_unit(double spot, double strike, double t, double vol, double r, double q, int32_t ot,
+               double shift, double z) {
+    const double st = gbm_terminal(spot, t, vol, r, q, z + shift);
+    const double y = z + shift;
+    const double w = std::exp(-shift * y + 0.5 * shift * shift);
+    return intrinsic(st, strike, ot) * w;
+}
+
+// pathwise_derivative.cpp:29-35 with model_concepts.h:108-111.
+double pw_unit(double spot, double strike, double t, double vol, double r, double q, int32_t ot,
+               double z) {
+    const double st = gbm_terminal(spot, t, vol, r, q, z);
+    if (ot == 0) return st > strike ? st / spot : 0.0;
+    return st < strike ? -(st / spot) : 0.0;
+}
+
+// likelihood_ratio.cpp:30-37 with model_concepts.h:121-128.
+double lr_unit(double spot, double strike, double t, double vol, double r, double q, int32_t ot,
+               double wclip, double z) {
+    const double st = gbm_terminal(spot, t, vol, r, q, z);
+    const double pay = ot == 0 ? std::max(st - strike, 0.0) : std::max(strike - st, 0.0);
+    const double sqrt_t = std::sqrt(t);
+    double score = z / (vol * sqrt_t * spot);
+    score = std::max(-wclip, std::min(wclip, score));
+    return pay * score;
+}
+
+// Heston path of heston_lr_delta.cpp:74-82 (model_concepts.h:205-226); the
+// draws come from `draw(j, &z1, &z2)`; sign = -1 gives the antithetic path.
+template <class Draw>
+double heston_lr_path(double spot, double r, double q, double v0, double kappa, double theta,
+                      double sigma, double rho, int32_t steps, double dt, double sdt, double sign,
+                      Draw&& draw, double* z1_first) {
+    const double rc = std::sqrt(std::max(1.0 - rho * rho, 0.0));
+    double s = spot, v = v0;
+    for (int32_t j = 0; j < steps; ++j) {
+        double z1, z2;
+        draw(j, &z1, &z2);
+        z1 *= sign;
+        z2 *= sign;
+        if (j == 0) *z1_first = z1;
+        const double vp = std::max(v, 0.0);
+        const double sv = std::sqrt(vp);
+        const double w1 = sdt * z1;
+        const double w2 = sdt * (rho * z1 + rc * z2);
+        const double s_new = s * std::exp((r - q - 0.5 * vp) * dt + sv * w1);
+        v = std::max(0.0, v + kappa * (theta - vp) * dt + sigma * sv * w2);
+        s = std::max(s_new, 1e-12);
+    }
+    return s;
+}
+
+bool heston_lr_ok(double spot, double strike, double t, double v0, double kappa, double theta,
+                  double sigma, double rho, int32_t ot, int32_t paths, int32_t steps,
+                  double wclip) {
+    // heston_lr_delta.cpp:47-54
+    return !(!type_ok(ot) || spot <= 0.0 || strike <= 0.0 || t < 0.0 || v0 < 0.0 ||
+             kappa < 0.0 || theta < 0.0 || sigma < 0.0 || rho < -1.0 || rho > 1.0 ||
+             !counts_ok(paths, steps) || wclip <= 0.0);
+}
+
+double heston_lr_intrinsic_delta(double spot, double strike, int32_t ot) {
+    // heston_lr_delta.cpp:57-61
+    if (ot == 0) return spot > strike ? 1.0 : 0.0;
+    return spot < strike ? -1.0 : 0.0;
+}
+
+}  // namespace
+
+// importance_sampling.cpp:11-36 (estimate_terminal, one draw per path).
+ORC_EXPORT double orc_importance_sampling(double spot, double strike, double t, double vol,
+                                          double r, double q, int32_t ot, int32_t paths,
+                                          double shift, uint64_t seed) {
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths <= 1 || !finite_bits(shift))
+        return qnan();
+    if (t <= kEps) return intrinsic(spot, strike, ot);
+    const double disc = std::exp(-r * t);
+    MtStream g(seed);
+    double sum = 0.0;
+    for (int32_t i = 0; i < paths; ++i) sum += is_unit(spot, strike, t, vol, r, q, ot, shift, g());
+    return disc * (sum / double(paths));
+}
+
+// pathwise_derivative.cpp:11-40.
+ORC_EXPORT double orc_pathwise_delta(double spot, double strike, double t, double vol, double r,
+                                     double q, int32_t ot, int32_t paths, uint64_t seed) {
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths < 256) return qnan();
+    if (t <= kEps || vol <= kEps) return deterministic_delta(spot, strike, t, r, q, ot);
+    const double disc = std::exp(-r * t);
+    MtStream g(seed);
+    double sum = 0.0;
+    for (int32_t i = 0; i < paths; ++i) sum += pw_unit(spot, strike, t, vol, r, q, ot, g());
+    return clamp_delta(disc * (sum / double(paths)), t, q);
+}
+
+// likelihood_ratio.cpp:12-42 (estimate_terminal_antithetic, mc_engine.h:32-53).
+ORC_EXPORT double orc_lr_delta(double spot, double strike, double t, double vol, double r,
+                               double q, int32_t ot, int32_t paths, uint64_t seed, double wclip) {
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths < 256 || !finite_bits(wclip) ||
+        wclip <= 0.0)
+        return qnan();
+    if (t <= kEps || vol <= kEps) return deterministic_delta(spot, strike, t, r, q, ot);
+    const double disc = std::exp(-r * t);
+    MtStream g(seed);
+    double sum = 0.0;
+    const int32_t pairs = paths / 2;
+    for (int32_t i = 0; i < pairs; ++i) {
+        const double z = g();
+        sum += lr_unit(spot, strike, t, vol, r, q, ot, wclip, z);
+        sum += lr_unit(spot, strike, t, vol, r, q, ot, wclip, -z);
+    }
+    if (paths & 1) sum += lr_unit(spot, strike, t, vol, r, q, ot, wclip, g());
+    return clamp_delta(disc * (sum / double(paths)), t, q);
+}
+
+// heston_lr_delta.cpp:41-132: antithetic pairs replay the pair's draws
+// negated; an odd path takes fresh draws; result disc * sum / paths.
+ORC_EXPORT double orc_heston_lr_delta(double spot, double strike, double t, double r, double q,
+                                      double v0, double kappa, double theta, double sigma,
+                                      double rho, int32_t ot, int32_t paths, int32_t steps,
+                                      uint64_t seed, double wclip) {
+    if (!heston_lr_ok(spot, strike, t, v0, kappa, theta, sigma, rho, ot, paths, steps, wclip))
+        return qnan();
+    if (t <= kEps) return heston_lr_intrinsic_delta(spot, strike, ot);
+    const double disc = std::exp(-r * t);
+    const double dt = t / double(steps);
+    const double sdt = std::sqrt(dt);
+    const double sqrt_v0 = std::sqrt(std::max(v0, 0.0));
+    MtStream g(seed);
+    std::vector<double> zs(size_t(2) * steps);
+    auto from_buf = [&](int32_t j, double* z1, double* z2) {
+        *z1 = zs[2 * j];
+        *z2 = zs[2 * j + 1];
+    };
+    auto score_of = [&](double z1) {
+        const double sc = sqrt_v0 > 1e-15 ? z1 / (sqrt_v0 * sdt * spot) : 0.0;
+        return std::max(-wclip, std::min(wclip, sc));
+    };
+    double sum = 0.0, z1f = 0.0, z1a = 0.0;
+    const int32_t pairs = paths / 2;
+    for (int32_t i = 0; i < pairs; ++i) {
+        for (auto& z : zs) z = g();
+        const double sf = heston_lr_path(spot, r, q, v0, kappa, theta, sigma, rho, steps, dt, sdt,
+                                         1.0, from_buf, &z1f);
+        const double sa = heston_lr_path(spot, r, q, v0, kappa, theta, sigma, rho, steps, dt, sdt,
+                                         -1.0, from_buf, &z1a);
+        const double score = score_of(z1f);
+        sum += intrinsic(sf, strike, ot) * score + intrinsic(sa, strike, ot) * (-score);
+    }
+    if (paths & 1) {
+        for (auto& z : zs) z = g();
+        const double s = heston_lr_path(spot, r, q, v0, kappa, theta, sigma, rho, steps, dt, sdt,
+                                        1.0, from_buf, &z1f);
+        sum += intrinsic(s, strike, ot) * score_of(z1f);
+    }
+    return disc * sum / double(paths);
+}
+
+// ---- the same four on the Philox stream (FP64), with standard errors ----
+// Units: importance sampling and pathwise delta one path each (terminal
+// draw k for path k); LR delta one antithetic pair (draw k for pair k, the
+// odd path draw `pairs`); Heston LR delta one pair (Philox path index k,
+// draws 2j and 2j+1 at step j; the odd path is path index `pairs`).
+ORC_EXPORT double orc_px_importance_sampling(double spot, double strike, double t, double vol,
+                                             double r, double q, int32_t ot, int32_t paths,
+                                             double shift, uint64_t seed, double* se) {
+    *se = qnan();
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths <= 1 || !finite_bits(shift))
+        return qnan();
+    *se = 0.0;
+    if (t <= kEps) return intrinsic(spot, strike, ot);
+    const double disc = std::exp(-r * t);
+    Moments mo;
+    for (int32_t i = 0; i < paths; ++i)
+        mo.add(is_unit(spot, strike, t, vol, r, q, ot, shift,
+                       philox_terminal_normal(seed, uint32_t(i))));
+    *se = se_of(mo, double(paths), disc);
+    return disc * (mo.s1 / double(paths));
+}
+
+ORC_EXPORT double orc_px_pathwise_delta(double spot, double strike, double t, double vol, double r,
+                                        double q, int32_t ot, int32_t paths, uint64_t seed,
+                                        double* se) {
+    *se = qnan();
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths < 256) return qnan();
+    *se = 0.0;
+    if (t <= kEps || vol <= kEps) return deterministic_delta(spot, strike, t, r, q, ot);
+    const double disc = std::exp(-r * t);
+    Moments mo;
+    for (int32_t i = 0; i < paths; ++i)
+        mo.add(pw_unit(spot, strike, t, vol, r, q, ot, philox_terminal_normal(seed, uint32_t(i))));
+    *se = se_of(mo, double(paths), disc);
+    return clamp_delta(disc * (mo.s1 / double(paths)), t, q);
+}
+
+ORC_EXPORT double orc_px_lr_delta(double spot, double strike, double t, double vol, double r,
+                                  double q, int32_t ot, int32_t paths, uint64_t seed, double wclip,
+                                  double* se) {
+    *se = qnan();
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths < 256 || !finite_bits(wclip) ||
+        wclip <= 0.0)
+        return qnan();
+    *se = 0.0;
+    if (t <= kEps || vol <= kEps) return deterministic_delta(spot, strike, t, r, q, ot);
+    const double disc = std::exp(-r * t);
+    const int32_t pairs = paths / 2;
+    double sum = 0.0;
+    Moments mo;
+    for (int32_t i = 0; i < pairs; ++i) {
+        const double z = philox_terminal_normal(seed, uint32_t(i));
+        const double a = lr_unit(spot, strike, t, vol, r, q, ot, wclip, z);
+        const double b = lr_unit(spot, strike, t, vol, r, q, ot, wclip, -z);
+        sum += a + b;
+        mo.add(0.5 * (a + b));
+    }
+    if (paths & 1)
+        sum += lr_unit(spot, strike, t, vol, r, q, ot, wclip,
+                       philox_terminal_normal(seed, uint32_t(pairs)));
+    *se = se_of(mo, double(pairs), disc);  // pair average as the unit, as for orc_px_terminal
+    return clamp_delta(disc * (sum / double(paths)), t, q);
+}
+
+ORC_EXPORT double orc_px_heston_lr_delta(double spot, double strike, double t, double r, double q,
+                                         double v0, double kappa, double theta, double sigma,
+                                         double rho, int32_t ot, int32_t paths, int32_t steps,
+                                         uint64_t seed, double wclip, double* se) {
+    *se = qnan();
+    if (!heston_lr_ok(spot, strike, t, v0, kappa, theta, sigma, rho, ot, paths, steps, wclip))
+        return qnan();
+    *se = 0.0;
+    if (t <= kEps) return heston_lr_intrinsic_delta(spot, strike, ot);
+    const double disc = std::exp(-r * t);
+    const double dt = t / double(steps);
+    const double sdt = std::sqrt(dt);
+    const double sqrt_v0 = std::sqrt(std::max(v0, 0.0));
+    auto score_of = [&](double z1) {
+        const double sc = sqrt_v0 > 1e-15 ? z1 / (sqrt_v0 * sdt * spot) : 0.0;
+        return std::max(-wclip, std::min(wclip, sc));
+    };
+    const int32_t pairs = paths / 2;
+    double sum = 0.0, z1f = 0.0, z1a = 0.0;
+    Moments mo;
+    for (int32_t i = 0; i <= pairs; ++i) {
+        if (i == pairs && !(paths & 1)) break;
+        auto draw = [&](int32_t j, double* z1, double* z2) {
+            *z1 = philox_path_normal(seed, uint32_t(i), uint32_t(2 * j));
+            *z2 = philox_path_normal(seed, uint32_t(i), uint32_t(2 * j + 1));
+        };
+        const double sf = heston_lr_path(spot, r, q, v0, kappa, theta, sigma, rho, steps, dt, sdt,
+                                         1.0, draw, &z1f);
+        const double score = score_of(z1f);
+        if (i == pairs) {
+            sum += intrinsic(sf, strike, ot) * score;
+            break;
+        }
+        const double sa = heston_lr_path(spot, r, q, v0, kappa, theta, sigma, rho, steps, dt, sdt,
+                                         -1.0, draw, &z1a);
+        const double u = intrinsic(sf, strike, ot) * score - intrinsic(sa, strike, ot) * score;
+        sum += u;
+        mo.add(u);
+    }
+    // unit = pair sum; the estimator is sum / paths = (pairs / paths) mean(unit)
+    *se = se_of(mo, double(pairs), disc) * pairs / double(paths);
+    return disc * sum / double(paths);
+}

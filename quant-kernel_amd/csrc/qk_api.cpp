@@ -108,6 +108,52 @@ int32_t heston_batch(const double* spot, const double* strike, const double* t, 
     return run(hb, n, out, se);
 }
 
+// GBM terminal estimators with a per-element parameter (shift / weight clip).
+int32_t gbm_aux_batch(int32_t method, const double* spot, const double* strike, const double* t,
+                      const double* vol, const double* r, const double* q, const int32_t* ot,
+                      const int32_t* paths, const double* aux, const uint64_t* seed, int32_t n,
+                      double* out, double* se) {
+    qk::HostBatch hb;
+    hb.uniform_method = method;
+    hb.spot = spot;
+    hb.strike = strike;
+    hb.t = t;
+    hb.vol = vol;
+    hb.r = r;
+    hb.q = q;
+    hb.ot = ot;
+    hb.paths = paths;
+    hb.seed = seed;
+    hb.aux = aux;
+    return run(hb, n, out, se);
+}
+
+int32_t heston_lr_batch(const double* spot, const double* strike, const double* t,
+                        const double* r, const double* q, const double* v0, const double* kappa,
+                        const double* theta, const double* sigma, const double* rho,
+                        const int32_t* ot, const int32_t* paths, const int32_t* steps,
+                        const uint64_t* seed, const double* wclip, int32_t n, double* out,
+                        double* se) {
+    qk::HostBatch hb;
+    hb.uniform_method = qk::M_HESTON_LR;
+    hb.spot = spot;
+    hb.strike = strike;
+    hb.t = t;
+    hb.r = r;
+    hb.q = q;
+    hb.v0 = v0;
+    hb.kappa = kappa;
+    hb.theta = theta;
+    hb.sigma = sigma;
+    hb.rho = rho;
+    hb.ot = ot;
+    hb.paths = paths;
+    hb.steps = steps;
+    hb.seed = seed;
+    hb.aux = wclip;
+    return run(hb, n, out, se);
+}
+
 double scalar_of(int32_t rc, double v) { return rc == QK_OK ? v : qk::qnan_d(); }
 
 }  // namespace
@@ -282,6 +328,174 @@ double qk_mcm_heston_monte_carlo_price(double spot, double strike, double t, dou
     double out = 0.0;
     const int32_t rc = heston_batch(&spot, &strike, &t, &r, &q, &v0, &kappa, &theta, &sigma, &rho,
                                     &option_type, &paths, &steps, &seed, 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- importance sampling (reference qk_api.cpp:918-925, 1055-1071) --------
+int32_t qk_mcm_importance_sampling_price_batch_se(const double* spot, const double* strike,
+                                                  const double* t, const double* vol,
+                                                  const double* r, const double* q,
+                                                  const int32_t* option_type,
+                                                  const int32_t* paths, const double* shift,
+                                                  const uint64_t* seed, int32_t n,
+                                                  double* out_prices, double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(shift), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_aux_batch(qk::M_IMPORTANCE, spot, strike, t, vol, r, q, option_type, paths, shift,
+                         seed, n, out_prices, out_stderr);
+}
+
+int32_t qk_mcm_importance_sampling_price_batch(const double* spot, const double* strike,
+                                               const double* t, const double* vol,
+                                               const double* r, const double* q,
+                                               const int32_t* option_type, const int32_t* paths,
+                                               const double* shift, const uint64_t* seed,
+                                               int32_t n, double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(shift), QK_P(seed),
+                     QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_aux_batch(qk::M_IMPORTANCE, spot, strike, t, vol, r, q, option_type, paths, shift,
+                         seed, n, out_prices, nullptr);
+}
+
+double qk_mcm_importance_sampling_price(double spot, double strike, double t, double vol,
+                                        double r, double q, int32_t option_type, int32_t paths,
+                                        double shift, uint64_t seed) {
+    double out = 0.0;
+    const int32_t rc = gbm_aux_batch(qk::M_IMPORTANCE, &spot, &strike, &t, &vol, &r, &q,
+                                     &option_type, &paths, &shift, &seed, 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- Heston likelihood-ratio delta (reference qk_api.cpp:1163-1199) -------
+int32_t qk_mcm_heston_lr_delta_batch_se(const double* spot, const double* strike, const double* t,
+                                        const double* r, const double* q, const double* v0,
+                                        const double* kappa, const double* theta,
+                                        const double* sigma, const double* rho,
+                                        const int32_t* option_type, const int32_t* paths,
+                                        const int32_t* steps, const uint64_t* seed,
+                                        const double* weight_clip, int32_t n,
+                                        double* out_prices, double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(r), QK_P(q), QK_P(v0), QK_P(kappa),
+                     QK_P(theta), QK_P(sigma), QK_P(rho), QK_P(option_type), QK_P(paths),
+                     QK_P(steps), QK_P(seed), QK_P(weight_clip), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return heston_lr_batch(spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type, paths,
+                           steps, seed, weight_clip, n, out_prices, out_stderr);
+}
+
+int32_t qk_mcm_heston_lr_delta_batch(const double* spot, const double* strike, const double* t,
+                                     const double* r, const double* q, const double* v0,
+                                     const double* kappa, const double* theta,
+                                     const double* sigma, const double* rho,
+                                     const int32_t* option_type, const int32_t* paths,
+                                     const int32_t* steps, const uint64_t* seed,
+                                     const double* weight_clip, int32_t n, double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(r), QK_P(q), QK_P(v0), QK_P(kappa),
+                     QK_P(theta), QK_P(sigma), QK_P(rho), QK_P(option_type), QK_P(paths),
+                     QK_P(steps), QK_P(seed), QK_P(weight_clip), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return heston_lr_batch(spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type, paths,
+                           steps, seed, weight_clip, n, out_prices, nullptr);
+}
+
+double qk_mcm_heston_lr_delta(double spot, double strike, double t, double r, double q,
+                              double v0, double kappa, double theta, double sigma, double rho,
+                              int32_t option_type, int32_t paths, int32_t steps, uint64_t seed,
+                              double weight_clip) {
+    double out = 0.0;
+    const int32_t rc = heston_lr_batch(&spot, &strike, &t, &r, &q, &v0, &kappa, &theta, &sigma,
+                                       &rho, &option_type, &paths, &steps, &seed, &weight_clip, 1,
+                                       &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- pathwise / likelihood-ratio delta (reference qk_api.cpp:1650-1667,
+//      1873-1907) ---------------------------------------------------------------
+int32_t qk_agm_pathwise_derivative_delta_batch_se(const double* spot, const double* strike,
+                                                  const double* t, const double* vol,
+                                                  const double* r, const double* q,
+                                                  const int32_t* option_type,
+                                                  const int32_t* paths, const uint64_t* seed,
+                                                  int32_t n, double* out_prices,
+                                                  double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_aux_batch(qk::M_PATHWISE, spot, strike, t, vol, r, q, option_type, paths, nullptr,
+                         seed, n, out_prices, out_stderr);
+}
+
+int32_t qk_agm_pathwise_derivative_delta_batch(const double* spot, const double* strike,
+                                               const double* t, const double* vol,
+                                               const double* r, const double* q,
+                                               const int32_t* option_type, const int32_t* paths,
+                                               const uint64_t* seed, int32_t n,
+                                               double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_aux_batch(qk::M_PATHWISE, spot, strike, t, vol, r, q, option_type, paths, nullptr,
+                         seed, n, out_prices, nullptr);
+}
+
+double qk_agm_pathwise_derivative_delta(double spot, double strike, double t, double vol,
+                                        double r, double q, int32_t option_type, int32_t paths,
+                                        uint64_t seed) {
+    double out = 0.0;
+    const int32_t rc = gbm_aux_batch(qk::M_PATHWISE, &spot, &strike, &t, &vol, &r, &q,
+                                     &option_type, &paths, nullptr, &seed, 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+int32_t qk_agm_likelihood_ratio_delta_batch_se(const double* spot, const double* strike,
+                                               const double* t, const double* vol,
+                                               const double* r, const double* q,
+                                               const int32_t* option_type, const int32_t* paths,
+                                               const uint64_t* seed, const double* weight_clip,
+                                               int32_t n, double* out_prices,
+                                               double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(weight_clip),
+                     QK_P(out_prices), QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_aux_batch(qk::M_LR, spot, strike, t, vol, r, q, option_type, paths, weight_clip,
+                         seed, n, out_prices, out_stderr);
+}
+
+int32_t qk_agm_likelihood_ratio_delta_batch(const double* spot, const double* strike,
+                                            const double* t, const double* vol, const double* r,
+                                            const double* q, const int32_t* option_type,
+                                            const int32_t* paths, const uint64_t* seed,
+                                            const double* weight_clip, int32_t n,
+                                            double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(weight_clip),
+                     QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_aux_batch(qk::M_LR, spot, strike, t, vol, r, q, option_type, paths, weight_clip,
+                         seed, n, out_prices, nullptr);
+}
+
+double qk_agm_likelihood_ratio_delta(double spot, double strike, double t, double vol, double r,
+                                     double q, int32_t option_type, int32_t paths, uint64_t seed,
+                                     double weight_clip) {
+    double out = 0.0;
+    const int32_t rc = gbm_aux_batch(qk::M_LR, &spot, &strike, &t, &vol, &r, &q, &option_type,
+                                     &paths, &weight_clip, &seed, 1, &out, nullptr);
     return scalar_of(rc, out);
 }
 

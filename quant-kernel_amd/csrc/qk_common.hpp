@@ -31,7 +31,21 @@ enum Method : int32_t {
     M_MILSTEIN = 3,    // milstein.cpp:11-28
     M_HESTON = 4,      // heston_monte_carlo.cpp:11-36
     M_LOCAL_VOL = 5,   // local_vol_monte_carlo.cpp:57-73
+    M_IMPORTANCE = 6,  // importance_sampling.cpp:11-36 (estimate_terminal, shifted draws)
+    M_PATHWISE = 7,    // pathwise_derivative.cpp:11-40 (estimate_terminal)
+    M_LR = 8,          // likelihood_ratio.cpp:12-42 (estimate_terminal_antithetic)
+    M_HESTON_LR = 9,   // heston_lr_delta.cpp:41-132 (antithetic Heston pairs)
 };
+
+// Terminal methods simulate S_T in one draw; the antithetic ones use a pair
+// (z, -z) per unit.
+QK_HD bool method_is_terminal(int32_t m) {
+    return m == M_STANDARD || m == M_ANTITHETIC || m == M_IMPORTANCE || m == M_PATHWISE ||
+           m == M_LR;
+}
+QK_HD bool method_is_antithetic_terminal(int32_t m) {
+    return m == M_STANDARD || m == M_ANTITHETIC || m == M_LR;
+}
 
 QK_HD double qnan_d() { return __builtin_nan(""); }
 
@@ -63,10 +77,11 @@ struct RawOption {
     double v0, kappa, theta, sigma, rho;
     int32_t ot, paths, steps;
     uint64_t seed;
+    double aux;  // importance sampling: shift; LR deltas: weight clip
 };
 
 // Prepared option: everything a path kernel and the finaliser need.
-// 160 bytes, 16-byte aligned, written by the prepare step.
+// 192 bytes, 16-byte aligned, written by the prepare step.
 struct alignas(16) OptionRec {
     int32_t family;      // Family
     int32_t method;      // Method
@@ -85,8 +100,14 @@ struct alignas(16) OptionRec {
     double c[8];         // model constants, see prepare_option()
     double closed;       // result when family == FAM_NONE
     double v0;           // Heston initial variance
+    // Greeks and importance sampling:
+    //  aux[0]: importance-sampling shift h, or the LR weight clip
+    //  aux[1]: e^{-qT}, the delta clamp (pathwise and LR delta)
+    //  aux[2]: LR score denominator: vol sqrt(T) spot (GBM) or
+    //          sqrt(v0) sqrt(dt) spot (Heston; 0 when sqrt(v0) <= 1e-15: score 0)
+    double aux[4];
 };
-static_assert(sizeof(OptionRec) == 160, "OptionRec layout");
+static_assert(sizeof(OptionRec) == 192, "OptionRec layout");
 
 // Model constants c[] by method (path kernels work on log2(S/spot) for
 // Heston and on S for the GBM schemes):
@@ -120,11 +141,13 @@ QK_HD int32_t prepare_option(const RawOption& in, int32_t tile_units, OptionRec&
     for (int i = 0; i < 8; ++i) rec.c[i] = 0.0;
     rec.closed = qnan_d();
     rec.v0 = 0.0;
+    for (int i = 0; i < 4; ++i) rec.aux[i] = 0.0;
 
     const double s = in.spot, k = in.strike, t = in.t, r = in.r, q = in.q;
     const int32_t ot = in.ot;
 
-    if (in.method == M_HESTON) {
+    if (in.method == M_HESTON || in.method == M_HESTON_LR) {
+        const bool lr = in.method == M_HESTON_LR;
         // heston_monte_carlo.cpp:18-24: no finiteness test on the inputs.
         // Non-finite Heston parameters give NaN here (the reference's result
         // for them is an artefact of std::max's NaN ordering).
@@ -133,12 +156,17 @@ QK_HD int32_t prepare_option(const RawOption& in, int32_t tile_units, OptionRec&
             !(in.sigma >= 0.0) || !(in.rho >= -1.0) || !(in.rho <= 1.0) || in.paths <= 1 ||
             in.steps <= 0)
             return FAM_NONE;
+        // heston_lr_delta.cpp:53: weight_clip <= 0 is invalid; +inf means no clipping
+        if (lr && !(in.aux > 0.0)) return FAM_NONE;
         if (!finite_d(s) || !finite_d(k) || !finite_d(t) || !finite_d(r) || !finite_d(q) ||
             !finite_d(in.v0) || !finite_d(in.kappa) || !finite_d(in.theta) ||
             !finite_d(in.sigma))
             return FAM_NONE;
         if (t <= kEps) {
-            rec.closed = intrinsic(s, k, ot);
+            if (lr)  // heston_lr_delta.cpp:57-61: the intrinsic delta
+                rec.closed = ot == QK_CALL ? (s > k ? 1.0 : 0.0) : (s < k ? -1.0 : 0.0);
+            else
+                rec.closed = intrinsic(s, k, ot);
             return FAM_NONE;
         }
         const double dt = t / double(in.steps);
@@ -154,9 +182,59 @@ QK_HD int32_t prepare_option(const RawOption& in, int32_t tile_units, OptionRec&
         const double one_m = 1.0 - in.rho * in.rho;
         rec.c[7] = sqrt(one_m > 0.0 ? one_m : 0.0);
         rec.v0 = in.v0;
-        rec.shift = intrinsic(s * exp((r - q) * t), k, ot);
         rec.family = FAM_STEPWISE;
+        if (lr) {
+            // unit = one antithetic pair; an odd last path is simulated apart
+            const double sv0 = sqrt(in.v0);
+            rec.aux[0] = in.aux;
+            rec.aux[2] = sv0 > 1e-15 ? sv0 * sdt * s : 0.0;
+            rec.units = in.paths / 2;
+        } else {
+            rec.shift = intrinsic(s * exp((r - q) * t), k, ot);
+            rec.units = in.paths;
+        }
+    } else if (in.method == M_PATHWISE || in.method == M_LR) {
+        // pathwise_derivative.cpp:15-22, likelihood_ratio.cpp:15-23
+        if (!common_ok(s, k, t, in.vol, r, q, ot) || in.paths < 256) return FAM_NONE;
+        if (in.method == M_LR && (!finite_d(in.aux) || in.aux <= 0.0)) return FAM_NONE;
+        if (t <= kEps || in.vol <= kEps) {
+            // adjoint_greeks/common/internal_util.h:22-31
+            const double fwd = s * exp((r - q) * t);
+            const double qf = exp(-q * t);
+            if (fabs(fwd - k) < kEps * k) rec.closed = ot == QK_CALL ? 0.5 * qf : -0.5 * qf;
+            else if (ot == QK_CALL) rec.closed = fwd > k ? qf : 0.0;
+            else rec.closed = fwd < k ? -qf : 0.0;
+            return FAM_NONE;
+        }
+        rec.disc = exp(-r * t);
+        rec.c[0] = (r - q - 0.5 * in.vol * in.vol) * t;
+        rec.c[1] = in.vol * sqrt(t);
+        rec.aux[1] = exp(-q * t);
+        rec.family = FAM_TERMINAL;
+        rec.steps = 1;
+        if (in.method == M_LR) {
+            rec.aux[0] = in.aux;
+            rec.aux[2] = in.vol * sqrt(t) * s;
+            rec.units = in.paths / 2;
+        } else {
+            rec.units = in.paths;
+        }
+    } else if (in.method == M_IMPORTANCE) {
+        // importance_sampling.cpp:15-20
+        if (!common_ok(s, k, t, in.vol, r, q, ot) || in.paths <= 1 || !finite_d(in.aux))
+            return FAM_NONE;
+        if (t <= kEps) {
+            rec.closed = intrinsic(s, k, ot);
+            return FAM_NONE;
+        }
+        rec.disc = exp(-r * t);
+        rec.c[0] = (r - q - 0.5 * in.vol * in.vol) * t;
+        rec.c[1] = in.vol * sqrt(t);
+        rec.aux[0] = in.aux;
+        rec.shift = intrinsic(s * exp((r - q) * t), k, ot);
+        rec.family = FAM_TERMINAL;
         rec.units = in.paths;
+        rec.steps = 1;
     } else if (in.method == M_STANDARD || in.method == M_ANTITHETIC) {
         // standard_monte_carlo.cpp:14-23, antithetic_variates.cpp:14-19
         if (!common_ok(s, k, t, in.vol, r, q, ot) || in.paths <= 1) return FAM_NONE;
@@ -202,16 +280,30 @@ QK_HD int32_t prepare_option(const RawOption& in, int32_t tile_units, OptionRec&
         rec.units = in.paths;
     }
     rec.ntiles = (rec.units + tile_units - 1) / tile_units;
-    if (rec.family == FAM_TERMINAL && rec.units == 0) rec.ntiles = 1;
+    if (rec.units == 0) rec.ntiles = 1;  // every simulated option owns at least one tile
     return rec.family;
 }
 
+// Validation-mode stream geometry: units that draw from the host stream
+// (paths, pairs, plus an odd path) and draws per unit.
+QK_HD int64_t stream_units(const OptionRec& rec) {
+    if (rec.family == FAM_TERMINAL)
+        return method_is_antithetic_terminal(rec.method)
+                   ? int64_t(rec.paths / 2) + (rec.paths & 1)
+                   : int64_t(rec.paths);
+    if (rec.family == FAM_STEPWISE)
+        return rec.method == M_HESTON_LR ? int64_t(rec.paths / 2) + (rec.paths & 1)
+                                         : int64_t(rec.paths);
+    return 0;
+}
+QK_HD int64_t stream_draws_per_unit(const OptionRec& rec) {
+    if (rec.family == FAM_STEPWISE)
+        return int64_t(rec.steps) * (rec.method == M_HESTON || rec.method == M_HESTON_LR ? 2 : 1);
+    return rec.family == FAM_TERMINAL ? 1 : 0;
+}
 // Number of host-stream normals an option consumes in the validation modes.
 QK_HD int64_t stream_normals(const OptionRec& rec) {
-    if (rec.family == FAM_TERMINAL) return int64_t(rec.paths / 2) + (rec.paths & 1);
-    if (rec.family == FAM_STEPWISE)
-        return int64_t(rec.paths) * rec.steps * (rec.method == M_HESTON ? 2 : 1);
-    return 0;
+    return stream_units(rec) * stream_draws_per_unit(rec);
 }
 
 // Per-tile partial sums of centred payoff units.

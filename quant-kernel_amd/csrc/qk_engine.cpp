@@ -106,7 +106,7 @@ int32_t method_of(const HostBatch& hb, int32_t i) {
     }
 }
 
-bool is_terminal_method(int32_t m) { return m == M_STANDARD || m == M_ANTITHETIC; }
+bool is_terminal_method(int32_t m) { return method_is_terminal(m); }
 
 size_t align64(size_t x) { return (x + 63) & ~size_t(63); }
 
@@ -153,8 +153,8 @@ int64_t tile_bound(const HostBatch& hb, int32_t n, const Device* d) {
         const int64_t p = std::max<int32_t>(hb.paths[i], 0);
         const int32_t m = method_of(hb, i);
         if (m < 0) continue;
-        if (is_terminal_method(m)) total += (p / 2 + tu_t - 1) / tu_t + 1;
-        else total += (p + tu_s - 1) / tu_s;
+        if (is_terminal_method(m)) total += (p + tu_t - 1) / tu_t + 1;
+        else total += (p + tu_s - 1) / tu_s + 1;
     }
     return total;
 }
@@ -225,8 +225,8 @@ static void fill_streams(const HostBatch& hb, int32_t n, const std::vector<Optio
                 const int64_t cnt = stream_normals(rc);
                 for (int64_t k = 0; k < cnt; ++k) base[k] = T(nd(eng));
             } else {
-                const int64_t P = rc.paths;
-                const int64_t D = int64_t(rc.steps) * (rc.method == M_HESTON ? 2 : 1);
+                const int64_t P = stream_units(rc);
+                const int64_t D = stream_draws_per_unit(rc);
                 for (int64_t p = 0; p < P; ++p)
                     for (int64_t dd = 0; dd < D; ++dd) base[dd * P + p] = T(nd(eng));
             }
@@ -243,19 +243,19 @@ static void fill_streams(const HostBatch& hb, int32_t n, const std::vector<Optio
 int32_t stage_host_batch(Device* d, const HostBatch& hb, int32_t n, BatchIn* bi_out,
                          const int32_t** n_dev_out, std::string* err, size_t* bytes_out) {
     const size_t ns = align64(size_t(n));
-    const double* dsrc[11] = {hb.spot, hb.strike, hb.t, hb.vol, hb.r, hb.q,
-                              hb.v0, hb.kappa, hb.theta, hb.sigma, hb.rho};
+    const double* dsrc[12] = {hb.spot, hb.strike, hb.t, hb.vol, hb.r, hb.q,
+                              hb.v0, hb.kappa, hb.theta, hb.sigma, hb.rho, hb.aux};
     const int32_t* isrc[4] = {hb.model_ids, hb.ot, hb.paths, hb.steps};
-    const size_t in_bytes = ns * (11 * 8 + 8 + 4 * 4) + 64;
+    const size_t in_bytes = ns * (12 * 8 + 8 + 4 * 4) + 64;
     QK_HIP(d->pin_in.ensure(in_bytes));
     QK_HIP(d->in.ensure(in_bytes));
     char* hp = static_cast<char*>(d->pin_in.p);
     char* dp = static_cast<char*>(d->in.p);
     BatchIn bi{};
-    const double** dfield[11] = {&bi.spot, &bi.strike, &bi.t, &bi.vol, &bi.r, &bi.q,
-                                 &bi.v0, &bi.kappa, &bi.theta, &bi.sigma, &bi.rho};
+    const double** dfield[12] = {&bi.spot, &bi.strike, &bi.t, &bi.vol, &bi.r, &bi.q,
+                                 &bi.v0, &bi.kappa, &bi.theta, &bi.sigma, &bi.rho, &bi.aux};
     size_t off = 0;
-    for (int k = 0; k < 11; ++k) {
+    for (int k = 0; k < 12; ++k) {
         if (dsrc[k]) {
             std::memcpy(hp + off, dsrc[k], size_t(n) * 8);
             *dfield[k] = reinterpret_cast<const double*>(dp + off);
@@ -408,6 +408,7 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
             o.paths = hb.paths[i];
             o.steps = hb.steps ? hb.steps[i] : 1;
             o.seed = hb.seed[i];
+            o.aux = hb.aux ? hb.aux[i] : 0.0;
             if (o.method < 0) {
                 hrec[i].family = FAM_NONE;
             } else {
@@ -459,8 +460,8 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
         uint64_t present = 0;
         const void* fields[] = {bi.method, bi.spot, bi.strike, bi.t, bi.vol, bi.r, bi.q, bi.v0,
                                 bi.kappa, bi.theta, bi.sigma, bi.rho, bi.ot, bi.paths, bi.steps,
-                                bi.seed};
-        for (int k = 0; k < 16; ++k) present |= uint64_t(fields[k] != nullptr) << k;
+                                bi.seed, bi.aux};
+        for (int k = 0; k < 17; ++k) present |= uint64_t(fields[k] != nullptr) << k;
         const std::vector<uint64_t> key = {
             uint64_t(n), uint64_t(family), uint64_t(bi.uniform_method), uint64_t(bi.method_is_model),
             present, uint64_t(st.lane_units), uint64_t(st.grid), uint64_t(tt.lane_units),

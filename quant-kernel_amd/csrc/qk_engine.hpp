@@ -20,6 +20,7 @@ struct HostBatch {
                  *rho = nullptr;
     const int32_t *ot = nullptr, *paths = nullptr, *steps = nullptr;
     const uint64_t* seed = nullptr;
+    const double* aux = nullptr;  // shift (importance sampling) / weight clip (LR deltas)
 };
 
 // Runs one batch on the GPU.  Returns a QK_* code; on failure *err names the

@@ -18,6 +18,7 @@ struct BatchIn {
     const double *v0, *kappa, *theta, *sigma, *rho;
     const int32_t *ot, *paths, *steps;
     const uint64_t* seed;
+    const double* aux;       // optional per-element shift / weight clip
     int32_t uniform_method;
     int32_t method_is_model;
     const uint8_t* active;   // optional: elements with active[i] == 0 are skipped (NaN)

@@ -40,6 +40,19 @@ __device__ __forceinline__ double q_fma(double a, double b, double c) { return _
 // std::max agree on every value that can occur here.
 __device__ __forceinline__ float q_max(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double q_max(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float q_min(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double q_min(double a, double b) { return __builtin_fmin(a, b); }
+
+// Likelihood-ratio score z / den clipped to [-clip, clip]
+// (likelihood_ratio.cpp:35-36, heston_lr_delta.cpp:88-91).
+template <typename R>
+__device__ __forceinline__ R lr_score(R z, R den, R inv_den, R clip) {
+    // FP64 keeps the reference's division; FP32 multiplies by the reciprocal
+    R sc;
+    if constexpr (sizeof(R) == 8) sc = z / den;
+    else sc = z * inv_den;
+    return q_max(-clip, q_min(clip, sc));
+}
 
 template <typename R>
 __device__ __forceinline__ R payoff(R s, R k, int32_t ot) {
@@ -139,6 +152,7 @@ __global__ __launch_bounds__(256) void k_prepare(BatchIn in, const int32_t* __re
         o.paths = in.paths_override > 0 ? in.paths_override : in.paths[i];
         o.steps = in.steps ? in.steps[i] : 1;
         o.seed = in.seed[i];
+        o.aux = in.aux ? in.aux[i] : 0.0;
         OptionRec rec;
         if (m < 0 || (in.active && !in.active[i])) {
             prepare_option(o, 1, rec);  // fills defaults
@@ -146,8 +160,7 @@ __global__ __launch_bounds__(256) void k_prepare(BatchIn in, const int32_t* __re
             rec.closed = qnan_d();
             rec.ntiles = 0;
         } else {
-            const bool term = (m == M_STANDARD || m == M_ANTITHETIC);
-            prepare_option(o, term ? tu_terminal : tu_stepwise, rec);
+            prepare_option(o, method_is_terminal(m) ? tu_terminal : tu_stepwise, rec);
         }
         if (rec.family == FAM_NONE) rec.ntiles = 0;
         if (stream_off) rec.stream_off = stream_off[i];
@@ -389,6 +402,107 @@ __device__ __forceinline__ float heston_path_px(uint32_t p, int32_t steps, const
     return y * k.ck;  // log2(S_T / spot)
 }
 
+// Antithetic Heston pair of heston_lr_delta.cpp:67-96: the forward path and
+// the path driven by the same draws negated; z1f is the forward path's first
+// z1 (the likelihood-ratio score's draw).
+template <typename R, class Src>
+__device__ __forceinline__ void heston_pair(const Src& src, uint32_t p, int32_t steps,
+                                            const HestonC<R>& k, R& xf, R& xa, R& z1f) {
+    R x = R(0), v = k.v0, y = R(0), w = k.v0;
+    const int32_t nb = steps >> 1;
+    for (int32_t b = 0; b < nb; ++b) {
+        const Quad<R> z = src.path_block(p, uint32_t(b), steps);
+        if (b == 0) z1f = z.z[0];
+        heston_step(k, x, v, z.z[0], z.z[1]);
+        heston_step(k, y, w, -z.z[0], -z.z[1]);
+        heston_step(k, x, v, z.z[2], z.z[3]);
+        heston_step(k, y, w, -z.z[2], -z.z[3]);
+    }
+    if (steps & 1) {
+        const Quad<R> z = src.path_block(p, uint32_t(nb), steps);
+        if (nb == 0) z1f = z.z[0];
+        heston_step(k, x, v, z.z[0], z.z[1]);
+        heston_step(k, y, w, -z.z[0], -z.z[1]);
+    }
+    xf = x;
+    xa = y;
+}
+
+// Production form of heston_pair: one Box-Muller evaluation drives both
+// paths (negating both normals is negating the radius).  zc returns
+// z1 / sqrt(2 ln 2) of the step.
+__device__ __forceinline__ void heston_px_pair_step(const HestonPx& k, float& y, float& v,
+                                                    float& ya, float& va, uint32_t wa, uint32_t wb,
+                                                    float& zc) {
+    const float u1 = __builtin_fmaf(float(wa), 0x1p-32f, 0x1p-33f);
+    const float rev = __builtin_bit_cast(float, (wb & 0x007FFFFFu) | 0x3F800000u);
+    const float rr = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1));
+    const float c1 = __builtin_amdgcn_cosf(rev);
+    const float c2 = __builtin_amdgcn_cosf(rev - k.phi);
+    zc = rr * c1;
+    const float t = __builtin_amdgcn_sqrtf(v) * rr;
+    y = __builtin_fmaxf(__builtin_fmaf(t, c1, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
+    v = __builtin_fmaxf(__builtin_fmaf(k.sgk * t, c2, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
+    const float ta = __builtin_amdgcn_sqrtf(va) * rr;
+    ya = __builtin_fmaxf(__builtin_fmaf(-ta, c1, ya + __builtin_fmaf(-k.h, va, k.a)), k.yf);
+    va = __builtin_fmaxf(__builtin_fmaf(-(k.sgk * ta), c2, __builtin_fmaf(k.omk, va, k.kth)),
+                         0.0f);
+}
+
+__device__ __forceinline__ void heston_pair_px(uint32_t p, int32_t steps, const HestonPx& k,
+                                               uint32_t k0, uint32_t k1, float& xf, float& xa,
+                                               float& z1f) {
+    float y = 0.0f, v = k.v0, ya = 0.0f, va = k.v0, zc = 0.0f, z0 = 0.0f;
+    const int32_t nb = steps >> 1;
+    for (int32_t b = 0; b < nb; ++b) {
+        const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
+        heston_px_pair_step(k, y, v, ya, va, w.x, w.y, zc);
+        if (b == 0) z0 = zc;
+        heston_px_pair_step(k, y, v, ya, va, w.z, w.w, zc);
+    }
+    if (steps & 1) {
+        const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
+        heston_px_pair_step(k, y, v, ya, va, w.x, w.y, zc);
+        if (nb == 0) z0 = zc;
+    }
+    xf = y * k.ck;
+    xa = ya * k.ck;
+    z1f = z0 * 1.1774100225154746910f;  // sqrt(2 ln 2)
+}
+
+// Heston constants of a record in both forms.
+template <typename R>
+__device__ __forceinline__ HestonC<R> heston_consts(const OptionRec& rec) {
+    HestonC<R> k;
+    k.a = R(rec.c[0]);
+    k.h = R(rec.c[1]);
+    k.c = R(rec.c[2]);
+    k.kdt = R(rec.c[3]);
+    k.kth = R(rec.c[4]);
+    k.sg = R(rec.c[5]);
+    k.rho = R(rec.c[6]);
+    k.rc = R(rec.c[7]);
+    k.xf = R(log2(1e-12 / rec.spot));
+    k.v0 = R(rec.v0);
+    return k;
+}
+
+__device__ __forceinline__ HestonPx heston_px_consts(const OptionRec& rec) {
+    HestonPx kp;
+    const double sq = 1.1774100225154746910;  // sqrt(2 ln 2)
+    const double ck = rec.c[2] * sq;          // sqrt(dt) log2e sqrt(2 ln 2)
+    kp.a = float(rec.c[0] / ck);
+    kp.h = float(rec.c[1] / ck);
+    kp.omk = float(1.0 - rec.c[3]);
+    kp.kth = float(rec.c[4]);
+    kp.sgk = float(rec.c[5] * sq);
+    kp.phi = float(atan2(rec.c[7], rec.c[6]) * 0.15915494309189533577);  // phi / 2 pi
+    kp.yf = float(log2(1e-12 / rec.spot) / ck);
+    kp.v0 = float(rec.v0);
+    kp.ck = float(ck);
+    return kp;
+}
+
 // ----------------------------------------------------------------------------
 // stepwise simulation (Euler, Milstein, local vol, Heston)
 // ----------------------------------------------------------------------------
@@ -418,33 +532,47 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
         StreamSrc<R> ss{stream + rec.stream_off, paths,
                         int64_t(steps) * (method == M_HESTON ? 2 : 1)};
 
-        double s1 = 0.0, s2 = 0.0;
-        if (method == M_HESTON) {
-            HestonC<R> k;
-            k.a = R(rec.c[0]);
-            k.h = R(rec.c[1]);
-            k.c = R(rec.c[2]);
-            k.kdt = R(rec.c[3]);
-            k.kth = R(rec.c[4]);
-            k.sg = R(rec.c[5]);
-            k.rho = R(rec.c[6]);
-            k.rc = R(rec.c[7]);
-            k.xf = R(log2(1e-12 / rec.spot));
-            k.v0 = R(rec.v0);
+        double s1 = 0.0, s2 = 0.0, extra = 0.0;
+        if (method == M_HESTON_LR) {
+            // units are antithetic pairs; an odd path count adds one forward
+            // path (Philox path index / stream unit = pairs) on lane 0 of the
+            // option's last tile
+            const HestonC<R> k = heston_consts<R>(rec);
             HestonPx kp;
-            if constexpr (MODE == SIM_PHILOX_F32) {
-                const double sq = 1.1774100225154746910;  // sqrt(2 ln 2)
-                const double ck = rec.c[2] * sq;          // sqrt(dt) log2e sqrt(2 ln 2)
-                kp.a = float(rec.c[0] / ck);
-                kp.h = float(rec.c[1] / ck);
-                kp.omk = float(1.0 - rec.c[3]);
-                kp.kth = float(rec.c[4]);
-                kp.sgk = float(rec.c[5] * sq);
-                kp.phi = float(atan2(rec.c[7], rec.c[6]) * 0.15915494309189533577);  // phi / 2 pi
-                kp.yf = float(log2(1e-12 / rec.spot) / ck);
-                kp.v0 = float(rec.v0);
-                kp.ck = float(ck);
+            if constexpr (MODE == SIM_PHILOX_F32) kp = heston_px_consts(rec);
+            const int32_t pairs = rec.units;
+            StreamSrc<R> sp{stream + rec.stream_off, int64_t(pairs) + (paths & 1),
+                            int64_t(steps) * 2};
+            const R den = R(rec.aux[2]), inv_den = R(1.0 / rec.aux[2]), clip = R(rec.aux[0]);
+            const bool no_score = rec.aux[2] == 0.0;  // sqrt(v0) <= 1e-15: score 0
+            const bool last = tile - rec.tile_first == rec.ntiles - 1;
+            for (int32_t j = 0; j <= lane_paths; ++j) {
+                int64_t p = first + threadIdx.x + int64_t(j) * kBlock;
+                if (j == lane_paths) {  // the odd path
+                    if (!((paths & 1) && last && threadIdx.x == 0)) break;
+                    p = pairs;
+                } else if (p >= pairs) {
+                    continue;
+                }
+                R xf, xa, z1;
+                if constexpr (MODE == SIM_PHILOX_F32)
+                    heston_pair_px(uint32_t(p), steps, kp, rec.seed_lo, rec.seed_hi, xf, xa, z1);
+                else heston_pair<R>(sp, uint32_t(p), steps, k, xf, xa, z1);
+                const R sc = no_score ? R(0) : lr_score(z1, den, inv_den, clip);
+                const R pf = payoff(spot * q_exp2(xf), strike, ot);
+                if (p == pairs) {
+                    extra = double(pf * sc);
+                    break;
+                }
+                const R pa = payoff(spot * q_exp2(xa), strike, ot);
+                const double f = double(pf * sc) - double(pa * sc);
+                s1 += f;
+                s2 = __builtin_fma(f, f, s2);
             }
+        } else if (method == M_HESTON) {
+            const HestonC<R> k = heston_consts<R>(rec);
+            HestonPx kp;
+            if constexpr (MODE == SIM_PHILOX_F32) kp = heston_px_consts(rec);
             for (int32_t j = 0; j < lane_paths; ++j) {
                 const int64_t p = first + threadIdx.x + int64_t(j) * kBlock;
                 if (p >= paths) break;
@@ -475,14 +603,80 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                 s2 = __builtin_fma(f, f, s2);
             }
         }
-        block_store_partial(s1, s2, 0.0, partials + tile, s_red);
+        // extra is only non-zero on thread 0, which writes the partial
+        block_store_partial(s1, s2, extra, partials + tile, s_red);
     }
 }
 
 // ----------------------------------------------------------------------------
-// terminal GBM, antithetic pairs (standard MC and antithetic variates)
-// S_T = spot * 2^(A + B z), A = (r-q-vol^2/2)T log2e, B = vol sqrt(T) log2e.
+// terminal GBM (mc_engine.h:32-75): S_T = spot * 2^(A + B z),
+// A = (r-q-vol^2/2)T log2e, B = vol sqrt(T) log2e.  One unit per draw:
+//   TK_PAIR  standard MC / antithetic variates: (pay(z) + pay(-z)) / 2
+//   TK_LR    likelihood-ratio delta: (pay(z) - pay(-z)) clip(z / den) / 2
+//            (likelihood_ratio.cpp:30-37; the mirror path's score is -score)
+//   TK_IS    importance sampling: pay(z + h) e^{-h(z+h) + h^2/2}
+//            (importance_sampling.cpp:26-34)
+//   TK_PW    pathwise delta: 1[S_T > K] S_T / spot (call), -1[S_T < K] S_T / spot
+//            (put) (pathwise_derivative.cpp:29-35)
 // ----------------------------------------------------------------------------
+enum TermKind : int32_t { TK_PAIR = 0, TK_LR = 1, TK_IS = 2, TK_PW = 3 };
+
+template <typename R>
+struct TermC {
+    R spot, strike, A, B;
+    R Ah;         // TK_IS: A + B h
+    R hn, hc;     // TK_IS: weight exponent -h log2e z - h^2/2 log2e
+    R den, inv_den, clip;  // TK_LR
+    int32_t ot;
+};
+
+template <typename R, int KIND>
+__device__ __forceinline__ double term_unit(const TermC<R>& k, R z) {
+    if constexpr (KIND == TK_PAIR) {
+        const R e = k.B * z;
+        const R up = payoff(k.spot * q_exp2(k.A + e), k.strike, k.ot);
+        const R dn = payoff(k.spot * q_exp2(k.A - e), k.strike, k.ot);
+        return 0.5 * (double(up) + double(dn));
+    } else if constexpr (KIND == TK_LR) {
+        const R e = k.B * z;
+        const R up = payoff(k.spot * q_exp2(k.A + e), k.strike, k.ot);
+        const R dn = payoff(k.spot * q_exp2(k.A - e), k.strike, k.ot);
+        const R sc = lr_score(z, k.den, k.inv_den, k.clip);
+        return 0.5 * (double(up * sc) - double(dn * sc));
+    } else if constexpr (KIND == TK_IS) {
+        const R st = k.spot * q_exp2(q_fma(k.B, z, k.Ah));
+        const R w = q_exp2(q_fma(k.hn, z, k.hc));
+        return double(payoff(st, k.strike, k.ot) * w);
+    } else {
+        const R e = q_exp2(q_fma(k.B, z, k.A));
+        const R st = k.spot * e;
+        if (k.ot == QK_CALL) return st > k.strike ? double(e) : 0.0;
+        return st < k.strike ? -double(e) : 0.0;
+    }
+}
+
+// One tile's units: lane handles Philox groups g0 + lane + j*kBlock (4 units each).
+template <typename R, int MODE, int KIND, class PS, class SS>
+__device__ __forceinline__ void term_tile(const TermC<R>& k, const PS& ph, const SS& ss,
+                                          int64_t g0, int32_t lane_groups, int32_t units,
+                                          double shift, double& s1, double& s2) {
+    for (int32_t j = 0; j < lane_groups; ++j) {
+        const int64_t g = g0 + threadIdx.x + int64_t(j) * kBlock;
+        if (g * 4 >= units) break;
+        Quad<R> z;
+        if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_block(uint32_t(g));
+        else z = ss.term_block(uint32_t(g));
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (g * 4 + c < units) {
+                const double f = term_unit<R, KIND>(k, z.z[c]) - shift;
+                s1 += f;
+                s2 = __builtin_fma(f, f, s2);
+            }
+        }
+    }
+}
+
 template <typename R, int MODE>
 __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __restrict__ recs,
                                                          const int32_t* __restrict__ tile_start,
@@ -500,40 +694,45 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
         if (tile >= total) break;
         const int32_t opt = find_option(tile_start, n, tile);
         const OptionRec& rec = recs[opt];
-        const int32_t pairs = rec.units, ot = rec.ot, paths = rec.paths;
+        const int32_t units = rec.units, paths = rec.paths, method = rec.method;
         const int32_t local_tile = tile - rec.tile_first;
         const int64_t g0 = int64_t(local_tile) * kBlock * lane_groups;  // first Philox group
-        const R spot = R(rec.spot), strike = R(rec.strike);
-        const R A = R(rec.c[0] * kLog2e), B = R(rec.c[1] * kLog2e);
+        TermC<R> k;
+        k.spot = R(rec.spot);
+        k.strike = R(rec.strike);
+        k.ot = rec.ot;
+        k.A = R(rec.c[0] * kLog2e);
+        k.B = R(rec.c[1] * kLog2e);
+        const double h = rec.aux[0];
+        k.Ah = R((rec.c[0] + rec.c[1] * h) * kLog2e);
+        k.hn = R(-h * kLog2e);
+        k.hc = R(-0.5 * h * h * kLog2e);
+        k.den = R(rec.aux[2]);
+        k.inv_den = R(1.0 / rec.aux[2]);
+        k.clip = R(rec.aux[0]);
         const double shift = rec.shift;
+        const bool anti = method_is_antithetic_terminal(method);
         PhiloxSrc ph{rec.seed_lo, rec.seed_hi};
-        StreamSrc<R> ss{stream + rec.stream_off, 1, int64_t(pairs) + (paths & 1)};
+        StreamSrc<R> ss{stream + rec.stream_off, 1, int64_t(units) + (anti ? (paths & 1) : 0)};
 
         double s1 = 0.0, s2 = 0.0;
-        for (int32_t j = 0; j < lane_groups; ++j) {
-            const int64_t g = g0 + threadIdx.x + int64_t(j) * kBlock;
-            if (g * 4 >= pairs) break;
-            Quad<R> z;
-            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_block(uint32_t(g));
-            else z = ss.term_block(uint32_t(g));
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if (g * 4 + c < pairs) {
-                    const R e = B * z.z[c];
-                    const R up = payoff(spot * q_exp2(A + e), strike, ot);
-                    const R dn = payoff(spot * q_exp2(A - e), strike, ot);
-                    const double f = 0.5 * (double(up) + double(dn)) - shift;
-                    s1 += f;
-                    s2 = __builtin_fma(f, f, s2);
-                }
-            }
-        }
+        if (method == M_LR)
+            term_tile<R, MODE, TK_LR>(k, ph, ss, g0, lane_groups, units, shift, s1, s2);
+        else if (method == M_IMPORTANCE)
+            term_tile<R, MODE, TK_IS>(k, ph, ss, g0, lane_groups, units, shift, s1, s2);
+        else if (method == M_PATHWISE)
+            term_tile<R, MODE, TK_PW>(k, ph, ss, g0, lane_groups, units, shift, s1, s2);
+        else
+            term_tile<R, MODE, TK_PAIR>(k, ph, ss, g0, lane_groups, units, shift, s1, s2);
+        // antithetic estimators: an odd path count adds one path with a fresh draw
         double extra = 0.0;
-        if ((paths & 1) && local_tile == rec.ntiles - 1 && threadIdx.x == 0) {
+        if (anti && (paths & 1) && local_tile == rec.ntiles - 1 && threadIdx.x == 0) {
             R z;
-            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_draw(uint32_t(pairs));
-            else z = ss.term_draw(uint32_t(pairs));
-            extra = double(payoff(spot * q_exp2(A + B * z), strike, ot));
+            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_draw(uint32_t(units));
+            else z = ss.term_draw(uint32_t(units));
+            const R pay = payoff(k.spot * q_exp2(k.A + k.B * z), k.strike, k.ot);
+            extra = method == M_LR ? double(pay * lr_score(z, k.den, k.inv_den, k.clip))
+                                   : double(pay);
         }
         block_store_partial(s1, s2, extra, partials + tile, s_red);
     }
@@ -573,17 +772,32 @@ __global__ __launch_bounds__(256) void k_finalize(const int32_t* __restrict__ n_
         double var = units > 1.0 ? (s2 - s1 * mean_d) / (units - 1.0) : 0.0;
         var = var > 0.0 ? var : 0.0;
         double price;
+        double se = rec.disc * sqrt(var / units);
+        const int32_t m = rec.method;
         if (rec.family == FAM_TERMINAL) {
-            const double sum = 2.0 * (rec.shift * units + s1) + extra;
+            // units are pairs (sum counts both paths) or single paths
+            const double sum = method_is_antithetic_terminal(m)
+                                   ? 2.0 * (rec.shift * units + s1) + extra
+                                   : rec.shift * units + s1;
             price = rec.disc * (sum / double(rec.paths));
-        } else if (rec.method == M_LOCAL_VOL) {
+            if (m == M_PATHWISE || m == M_LR) {
+                // adjoint_greeks/common/internal_util.h:47-50
+                const double qf = rec.aux[1];
+                price = fmin(qf, fmax(-qf, price));
+            }
+        } else if (m == M_LOCAL_VOL) {
             const double sum = rec.shift * units + s1;
             price = rec.disc * sum / double(rec.paths);  // local_vol_monte_carlo.cpp:47
+        } else if (m == M_HESTON_LR) {
+            // heston_lr_delta.cpp:131: disc * sum / paths, units are pair sums
+            const double sum = rec.shift * units + s1 + extra;
+            price = rec.disc * sum / double(rec.paths);
+            se *= units / double(rec.paths);
         } else {
             price = rec.disc * (rec.shift + mean_d);
         }
         out_price[i] = price;
-        out_se[i] = rec.disc * sqrt(var / units);
+        out_se[i] = se;
     }
 }
 

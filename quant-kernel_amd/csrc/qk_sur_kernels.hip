@@ -40,6 +40,7 @@ __global__ __launch_bounds__(256) void k_sur_prepare(BatchIn in, int n, int Mpad
         o.paths = in.paths[i];
         o.steps = in.steps[i];
         o.seed = in.seed[i];
+        o.aux = 0.0;
         OptionRec rec;
         act_fb[i] = 0;
         act_spot[i] = 0;

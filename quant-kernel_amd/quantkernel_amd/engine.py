@@ -106,6 +106,18 @@ _SIGNATURES = {
     "qk_mcm_heston_monte_carlo_price": (_D, [_D] * 10 + [_I, _I, _I, _U]),
     "qk_mcm_heston_monte_carlo_price_batch": (_I, [_PD] * 10 + [_PI, _PI, _PI, _PU, _I, _PD]),
     "qk_mcm_heston_monte_carlo_price_batch_se": (_I, [_PD] * 10 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_mcm_importance_sampling_price": (_D, [_D] * 6 + [_I, _I, _D, _U]),
+    "qk_mcm_importance_sampling_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PD, _PU, _I, _PD]),
+    "qk_mcm_importance_sampling_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PD, _PU, _I, _PD, _PD]),
+    "qk_mcm_heston_lr_delta": (_D, [_D] * 10 + [_I, _I, _I, _U, _D]),
+    "qk_mcm_heston_lr_delta_batch": (_I, [_PD] * 10 + [_PI, _PI, _PI, _PU, _PD, _I, _PD]),
+    "qk_mcm_heston_lr_delta_batch_se": (_I, [_PD] * 10 + [_PI, _PI, _PI, _PU, _PD, _I, _PD, _PD]),
+    "qk_agm_pathwise_derivative_delta": (_D, [_D] * 6 + [_I, _I, _U]),
+    "qk_agm_pathwise_derivative_delta_batch": (_I, [_PD] * 6 + [_PI, _PI, _PU, _I, _PD]),
+    "qk_agm_pathwise_derivative_delta_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_agm_likelihood_ratio_delta": (_D, [_D] * 6 + [_I, _I, _U, _D]),
+    "qk_agm_likelihood_ratio_delta_batch": (_I, [_PD] * 6 + [_PI, _PI, _PU, _PD, _I, _PD]),
+    "qk_agm_likelihood_ratio_delta_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PU, _PD, _I, _PD, _PD]),
     "qk_mcm_mixed_price_batch": (_I, [_PI] + [_PD] * 11 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
     "qk_mcm_mixed_price_batch_device": (_I, [ct.c_void_p] * 16 + [_I, ct.c_void_p, ct.c_void_p]),
     "qk_mc_set_rng_mode": (_I, [_I]),
@@ -411,8 +423,8 @@ class QuantKernel:
 
     # ---- standard / antithetic (engine.py:1352-1393, 1466-1483, 1616-1647) -
     def standard_monte_carlo_price(self, spot, strike, t, vol, r, q, option_type, paths=100000, seed=42):
-        return float(self._lib.qk_mcm_standard_monte_carlo_price(
-            spot, strike, t, vol, r, q, int(option_type), int(paths), int(seed)))
+        return self._call_checked("qk_mcm_standard_monte_carlo_price", spot, strike, t, vol, r, q,
+                                  int(option_type), int(paths), int(seed))
 
     def standard_monte_carlo_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, seed,
                                          return_stderr=False):
@@ -424,8 +436,8 @@ class QuantKernel:
         return self._call_batch("qk_mcm_standard_monte_carlo_price_batch", *a, p, s)
 
     def antithetic_variates_price(self, spot, strike, t, vol, r, q, option_type, paths=100000, seed=42):
-        return float(self._lib.qk_mcm_antithetic_variates_price(
-            spot, strike, t, vol, r, q, int(option_type), int(paths), int(seed)))
+        return self._call_checked("qk_mcm_antithetic_variates_price", spot, strike, t, vol, r, q,
+                                  int(option_type), int(paths), int(seed))
 
     def antithetic_variates_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, seed):
         a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
@@ -434,9 +446,21 @@ class QuantKernel:
         return self._call_batch("qk_mcm_antithetic_variates_price_batch", *a, p, s)
 
     # ---- stepwise GBM (engine.py:1394-1411, 1484-1517, 1777-1823) ---------
+    def _call_checked(self, fn_name: str, *args) -> float:
+        """Scalar call; NaN raises ValueError as in the reference (engine.py:93-97)."""
+        before = self.get_last_error()
+        out = float(getattr(self._lib, fn_name)(*args))
+        if math.isnan(out):
+            msg = f"{fn_name} returned NaN; check inputs."
+            after = self.get_last_error()
+            if after and after != before:
+                msg += f" ({after})"
+            raise ValueError(msg)
+        return out
+
     def _stepwise_scalar(self, fn, spot, strike, t, vol, r, q, option_type, paths, steps, seed):
-        return float(getattr(self._lib, fn)(spot, strike, t, vol, r, q, int(option_type), int(paths),
-                                            int(steps), int(seed)))
+        return self._call_checked(fn, spot, strike, t, vol, r, q, int(option_type), int(paths), int(steps),
+                                  int(seed))
 
     def _stepwise_batch(self, fn, spot, strike, t, vol, r, q, option_type, paths, steps, seed, se):
         a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
@@ -496,9 +520,8 @@ class QuantKernel:
     # ---- Heston (engine.py:1648-1718) ------------------------------------
     def heston_monte_carlo_price(self, spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type,
                                  paths=50000, steps=100, seed=42):
-        return float(self._lib.qk_mcm_heston_monte_carlo_price(
-            spot, strike, t, r, q, v0, kappa, theta, sigma, rho, int(option_type), int(paths), int(steps),
-            int(seed)))
+        return self._call_checked("qk_mcm_heston_monte_carlo_price", spot, strike, t, r, q, v0, kappa, theta,
+                                  sigma, rho, int(option_type), int(paths), int(steps), int(seed))
 
     def heston_monte_carlo_price_batch(self, spot, strike, t, r, q, v0, kappa, theta, sigma, rho,
                                        option_type, paths, steps, seed, return_stderr=False):
@@ -516,6 +539,75 @@ class QuantKernel:
         return self._call_batch("qk_mcm_heston_monte_carlo_price_batch", *f, ot, p, st, s)
 
     # ---- mixed-model batch (ABI 2.13) ------------------------------------
+    # ---- importance sampling and Greeks on the MC engine (engine.py:965-982,
+    #      1448-1455, 1583-1598, 1720-1775, 2090-2121) ------------------------
+    def importance_sampling_price(self, spot, strike, t, vol, r, q, option_type, paths, shift, seed=42):
+        return self._call_checked("qk_mcm_importance_sampling_price", spot, strike, t, vol, r, q,
+                                  int(option_type), int(paths), float(shift), int(seed))
+
+    def importance_sampling_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, shift, seed,
+                                        return_stderr=False):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        p, h, s = (self._as_i32_array(paths, "paths"), self._as_f64_array(shift, "shift"),
+                   self._as_u64_array(seed, "seed"))
+        self._check_same_length(a[0].shape[0], paths=p, shift=h, seed=s)
+        fn = "qk_mcm_importance_sampling_price_batch"
+        if return_stderr:
+            return self._call_batch(fn + "_se", *a, p, h, s, with_se=True)
+        return self._call_batch(fn, *a, p, h, s)
+
+    def pathwise_derivative_delta(self, spot, strike, t, vol, r, q, option_type, paths=20000, seed=42):
+        return self._call_checked("qk_agm_pathwise_derivative_delta", spot, strike, t, vol, r, q,
+                                  int(option_type), int(paths), int(seed))
+
+    def pathwise_derivative_delta_batch(self, spot, strike, t, vol, r, q, option_type, paths, seed,
+                                        return_stderr=False):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        p, s = self._as_i32_array(paths, "paths"), self._as_u64_array(seed, "seed")
+        self._check_same_length(a[0].shape[0], paths=p, seed=s)
+        fn = "qk_agm_pathwise_derivative_delta_batch"
+        if return_stderr:
+            return self._call_batch(fn + "_se", *a, p, s, with_se=True)
+        return self._call_batch(fn, *a, p, s)
+
+    def likelihood_ratio_delta(self, spot, strike, t, vol, r, q, option_type, paths=20000, seed=42,
+                               weight_clip=6.0):
+        return self._call_checked("qk_agm_likelihood_ratio_delta", spot, strike, t, vol, r, q,
+                                  int(option_type), int(paths), int(seed), float(weight_clip))
+
+    def likelihood_ratio_delta_batch(self, spot, strike, t, vol, r, q, option_type, paths, seed, weight_clip,
+                                     return_stderr=False):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        p, s, w = (self._as_i32_array(paths, "paths"), self._as_u64_array(seed, "seed"),
+                   self._as_f64_array(weight_clip, "weight_clip"))
+        self._check_same_length(a[0].shape[0], paths=p, seed=s, weight_clip=w)
+        fn = "qk_agm_likelihood_ratio_delta_batch"
+        if return_stderr:
+            return self._call_batch(fn + "_se", *a, p, s, w, with_se=True)
+        return self._call_batch(fn, *a, p, s, w)
+
+    def heston_lr_delta(self, spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type, paths, steps,
+                        seed=42, weight_clip=6.0):
+        return self._call_checked("qk_mcm_heston_lr_delta", spot, strike, t, r, q, v0, kappa, theta, sigma, rho,
+                                  int(option_type), int(paths), int(steps), int(seed), float(weight_clip))
+
+    def heston_lr_delta_batch(self, spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type, paths,
+                              steps, seed, weight_clip, return_stderr=False):
+        f = [self._as_f64_array(x, nm) for x, nm in
+             zip((spot, strike, t, r, q, v0, kappa, theta, sigma, rho),
+                 ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho"))]
+        n = f[0].shape[0]
+        ot, p, st, s, w = (self._as_i32_array(option_type, "option_type"), self._as_i32_array(paths, "paths"),
+                           self._as_i32_array(steps, "steps"), self._as_u64_array(seed, "seed"),
+                           self._as_f64_array(weight_clip, "weight_clip"))
+        self._check_same_length(n, **{nm: x for nm, x in zip(
+            ("strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho"), f[1:])},
+            option_type=ot, paths=p, steps=st, seed=s, weight_clip=w)
+        fn = "qk_mcm_heston_lr_delta_batch"
+        if return_stderr:
+            return self._call_batch(fn + "_se", *f, ot, p, st, s, w, with_se=True)
+        return self._call_batch(fn, *f, ot, p, st, s, w)
+
     def mixed_price_batch(self, model, spot, strike, t, vol, r, q, v0, kappa, theta, sigma, rho, option_type,
                           paths, steps, seed):
         m = self._as_i32_array(model, "model")

@@ -31,6 +31,14 @@ _ORC = {
     "orc_bsm_as": (D, [D] * 6 + [I]),
     "orc_heston_cf": (D, [D] * 10 + [I, I, D]),
     "orc_crr": (D, [D] * 6 + [I, I, I]),
+    "orc_importance_sampling": (D, [D] * 6 + [I, I, D, U]),
+    "orc_pathwise_delta": (D, [D] * 6 + [I, I, U]),
+    "orc_lr_delta": (D, [D] * 6 + [I, I, U, D]),
+    "orc_heston_lr_delta": (D, [D] * 10 + [I, I, I, U, D]),
+    "orc_px_importance_sampling": (D, [D] * 6 + [I, I, D, U, PD]),
+    "orc_px_pathwise_delta": (D, [D] * 6 + [I, I, U, PD]),
+    "orc_px_lr_delta": (D, [D] * 6 + [I, I, U, D, PD]),
+    "orc_px_heston_lr_delta": (D, [D] * 10 + [I, I, I, U, D, PD]),
     "orc_heston_batch": (None, [PD] * 10 + [PI, PI, PI, PU, I, PD]),
     "orc_standard_mc_batch": (None, [PD] * 6 + [PI, PI, PU, I, PD]),
 }
@@ -79,6 +87,10 @@ _REF = {
     "qk_mcm_heston_monte_carlo_price_batch": (I, [PD] * 10 + [PI, PI, PI, PU, I, PD]),
     "qk_mcm_standard_monte_carlo_price_batch": (I, [PD] * 6 + [PI, PI, PU, I, PD]),
     "qk_cf_black_scholes_merton_price": (D, [D] * 6 + [I]),
+    "qk_mcm_importance_sampling_price": (D, [D] * 6 + [I, I, D, U]),
+    "qk_agm_pathwise_derivative_delta": (D, [D] * 6 + [I, I, U]),
+    "qk_agm_likelihood_ratio_delta": (D, [D] * 6 + [I, I, U, D]),
+    "qk_mcm_heston_lr_delta": (D, [D] * 10 + [I, I, I, U, D]),
 }
 
 

@@ -157,7 +157,9 @@ def test_no_device_means_runtime_init_not_fallback(qk):
     argv = [_ptr(a[nm]) for nm in ORDER + ["out_prices"]]
     assert _call("qk_mcm_euler_maruyama_price_batch", qk, argv, 2) == -4
     assert "device" in qk.get_last_error()
-    assert np.isnan(qk.euler_maruyama_price(100, 100, 1, .2, .05, 0, 0, 1000, 10, 1))
+    with pytest.raises(ValueError, match="NaN"):  # scalar NaN raises, as in the reference wrapper
+        qk.euler_maruyama_price(100, 100, 1, .2, .05, 0, 0, 1000, 10, 1)
+    assert np.isnan(qk._lib.qk_mcm_euler_maruyama_price(100, 100, 1, .2, .05, 0, 0, 1000, 10, 1))
 
 
 def test_c_consumer_compiles_and_links(tmp_path):
