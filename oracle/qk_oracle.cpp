@@ -138,6 +138,29 @@ double philox_path_normal(uint64_t seed, uint32_t p, uint32_t d) {
     return z[d & 3u];
 }
 
+// Heston (production) stream: step j of path p uses slot j%3 of block
+// (p, j/3, 0, 0).  u1 = (2m + 1) 2^-24 with m the low 23 bits of word j%3;
+// the angle fraction is the low 23 bits of w3 (slot 0), or 15 bits built from
+// bits 24-30 of w0 and 24-31 of w1 (slot 1) / of w2 and w3 (slot 2), over
+// 2^23.  (z1, z2) = sqrt(-2 ln u1) (cos 2 pi f, sin 2 pi f) in FP64.
+void philox_heston_normals(uint64_t seed, uint32_t p, uint32_t j, double* z1, double* z2) {
+    const uint32_t in[4] = {p, j / 3u, 0u, 0u};
+    const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    uint32_t w[4];
+    philox10(in, key, w);
+    const uint32_t s = j % 3u;
+    const double u1 = (2.0 * double(w[s] & 0x007FFFFFu) + 1.0) * 0x1p-24;
+    uint32_t fb;
+    if (s == 0) fb = w[3] & 0x007FFFFFu;
+    else if (s == 1) fb = ((w[0] >> 24) & 0x7Fu) << 16 | (w[1] >> 24) << 8;
+    else fb = ((w[2] >> 24) & 0x7Fu) << 16 | (w[3] >> 24) << 8;
+    const double f = double(fb) * (1.0 / 8388608.0);
+    const double rad = std::sqrt(-2.0 * std::log(u1));
+    const double ang = 6.283185307179586476925286766559 * f;
+    *z1 = rad * std::cos(ang);
+    *z2 = rad * std::sin(ang);
+}
+
 // Terminal estimators: draw k: block (k/4, 0), tag 1, component k%4.
 double philox_terminal_normal(uint64_t seed, uint32_t k) {
     double z[4];
@@ -477,6 +500,11 @@ ORC_EXPORT double orc_philox_path_normal(uint64_t seed, uint32_t path, uint32_t 
     return philox_path_normal(seed, path, draw);
 }
 
+ORC_EXPORT void orc_philox_heston_normals(uint64_t seed, uint32_t path, uint32_t step, double* z1,
+                                          double* z2) {
+    philox_heston_normals(seed, path, step, z1, z2);
+}
+
 ORC_EXPORT double orc_philox_terminal_normal(uint64_t seed, uint32_t k) {
     return philox_terminal_normal(seed, k);
 }
@@ -558,8 +586,8 @@ ORC_EXPORT double orc_px_heston(double spot, double strike, double t, double r, 
     for (int32_t i = 0; i < paths; ++i) {
         double s = spot, v = v0;
         for (int32_t j = 0; j < steps; ++j) {
-            const double z1 = philox_path_normal(seed, uint32_t(i), uint32_t(2 * j));
-            const double z2 = philox_path_normal(seed, uint32_t(i), uint32_t(2 * j + 1));
+            double z1, z2;
+            philox_heston_normals(seed, uint32_t(i), uint32_t(j), &z1, &z2);
             const double vp = std::max(v, 0.0);
             const double sv = std::sqrt(vp);
             const double w1 = sdt * z1;
@@ -1016,8 +1044,7 @@ ORC_EXPORT double orc_px_heston_lr_delta(double spot, double strike, double t, d
     for (int32_t i = 0; i <= pairs; ++i) {
         if (i == pairs && !(paths & 1)) break;
         auto draw = [&](int32_t j, double* z1, double* z2) {
-            *z1 = philox_path_normal(seed, uint32_t(i), uint32_t(2 * j));
-            *z2 = philox_path_normal(seed, uint32_t(i), uint32_t(2 * j + 1));
+            philox_heston_normals(seed, uint32_t(i), uint32_t(j), z1, z2);
         };
         const double sf = heston_lr_path(spot, r, q, v0, kappa, theta, sigma, rho, steps, dt, sdt,
                                          1.0, draw, &z1f);

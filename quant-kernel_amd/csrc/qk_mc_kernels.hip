@@ -361,43 +361,71 @@ __device__ __forceinline__ R heston_path(const Src& src, uint32_t p, int32_t ste
     return x;
 }
 
-// Production Heston path: the same scheme and the same Philox draws as
-// heston_path, rearranged to issue fewer instructions.  With Box-Muller
-// radius r = sqrt(2 ln 2) * sqrt(-log2 u1) and angle theta,
+// Production Heston path: the scheme of heston_path on the Philox stream,
+// rearranged to issue fewer instructions.  With Box-Muller radius
+// r = sqrt(2 ln 2) * sqrt(-log2 u1) and angle theta,
 //   z1              = r cos(theta)
 //   rho z1 + rho' z2 = r cos(theta - phi),   cos phi = rho, sin phi = rho'
 // and the log-spot is kept in units of c sqrt(2 ln 2) (c = sqrt(dt) log2e),
 // so per step: y <- max(y + (a - h v)/cK + sqrt(v) r' cos(theta), floor/cK),
 // v <- max(0, (1 - kappa dt) v + kappa theta dt + sigma sqrt(dt) sqrt(2 ln 2)
 // sqrt(v) r' cos(theta - phi)), with r' = sqrt(-log2 u1).
+//
+// Heston stream: Philox block (p, j/3, 0, 0) drives steps 3(j/3) .. 3(j/3)+2
+// of path p.  Slot s of the block (words w0..w3):
+//   u1  = (2m + 1) 2^-24, m = low 23 bits of w_s (exact in FP32, in (0,1));
+//   angle fraction (revolutions) from bits disjoint from all three m:
+//     slot 0: low 23 bits of w3;
+//     slot 1: bits 24-30 of w0 and 24-31 of w1 (15 bits);
+//     slot 2: bits 24-30 of w2 and 24-31 of w3 (15 bits).
+// One block per three steps instead of two saves a third of the Philox work;
+// the radius is capped at sqrt(48 ln 2) = 5.77 sigma.
 struct HestonPx {
     float a, h, omk, kth, sgk, phi, yf, v0, ck;
 };
 
-__device__ __forceinline__ void heston_px_step(const HestonPx& k, float& y, float& v, uint32_t wa,
-                                               uint32_t wb) {
-    const float u1 = __builtin_fmaf(float(wa), 0x1p-32f, 0x1p-33f);
-    const float rev = __builtin_bit_cast(float, (wb & 0x007FFFFFu) | 0x3F800000u);
-    const float rr = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1));
+// max(x, f) for finite x, f as v_med3_f32(x, f, FLT_MAX): unlike fmaxf it
+// needs no canonicalisation of the loop-invariant floor inside the loop.
+__device__ __forceinline__ float floor_max(float x, float f) {
+    return __builtin_amdgcn_fmed3f(x, f, 0x1.fffffep+127f);
+}
+
+__device__ __forceinline__ float u23(uint32_t w) {
+    return __builtin_bit_cast(float, (w & 0x007FFFFFu) | 0x3F800000u) - 0x1.fffffep-1f;
+}
+__device__ __forceinline__ float rev23(uint32_t bits) {  // 1 + fraction, in [1, 2)
+    return __builtin_bit_cast(float, (bits & 0x007FFFFFu) | 0x3F800000u);
+}
+// bytes {0, w_hi.byte3, w_lo.byte3, 0} (v_perm_b32)
+__device__ __forceinline__ uint32_t top_bytes(uint32_t w_hi, uint32_t w_lo) {
+    return __builtin_amdgcn_perm(w_hi, w_lo, 0x0C07030Cu);
+}
+
+// sqrt(v) r' = sqrt(-v log2 u1): one square root per step instead of two.
+__device__ __forceinline__ void heston_px_step(const HestonPx& k, float& y, float& v, float u1,
+                                               float rev) {
+    const float lg = __builtin_amdgcn_logf(u1);  // < 0
     const float c1 = __builtin_amdgcn_cosf(rev);
     const float c2 = __builtin_amdgcn_cosf(rev - k.phi);
-    const float t = __builtin_amdgcn_sqrtf(v) * rr;
-    y = __builtin_fmaxf(__builtin_fmaf(t, c1, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
+    const float t = __builtin_amdgcn_sqrtf(-(v * lg));
+    y = floor_max(__builtin_fmaf(t, c1, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
     v = __builtin_fmaxf(__builtin_fmaf(k.sgk * t, c2, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
 }
 
 __device__ __forceinline__ float heston_path_px(uint32_t p, int32_t steps, const HestonPx& k,
                                                 uint32_t k0, uint32_t k1) {
     float y = 0.0f, v = k.v0;
-    const int32_t nb = steps >> 1;
+    const int32_t nb = steps / 3, rem = steps - 3 * nb;
     for (int32_t b = 0; b < nb; ++b) {
         const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
-        heston_px_step(k, y, v, w.x, w.y);
-        heston_px_step(k, y, v, w.z, w.w);
+        heston_px_step(k, y, v, u23(w.x), rev23(w.w));
+        heston_px_step(k, y, v, u23(w.y), rev23(top_bytes(w.x, w.y)));
+        heston_px_step(k, y, v, u23(w.z), rev23(top_bytes(w.z, w.w)));
     }
-    if (steps & 1) {
+    if (rem) {
         const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
-        heston_px_step(k, y, v, w.x, w.y);
+        heston_px_step(k, y, v, u23(w.x), rev23(w.w));
+        if (rem > 1) heston_px_step(k, y, v, u23(w.y), rev23(top_bytes(w.x, w.y)));
     }
     return y * k.ck;  // log2(S_T / spot)
 }
@@ -429,22 +457,19 @@ __device__ __forceinline__ void heston_pair(const Src& src, uint32_t p, int32_t 
 }
 
 // Production form of heston_pair: one Box-Muller evaluation drives both
-// paths (negating both normals is negating the radius).  zc returns
-// z1 / sqrt(2 ln 2) of the step.
+// paths (negating both normals is negating the radius).  lg and c1 return
+// log2 u1 and cos(theta) of the step (z1 = sqrt(2 ln 2) sqrt(-lg) c1).
 __device__ __forceinline__ void heston_px_pair_step(const HestonPx& k, float& y, float& v,
-                                                    float& ya, float& va, uint32_t wa, uint32_t wb,
-                                                    float& zc) {
-    const float u1 = __builtin_fmaf(float(wa), 0x1p-32f, 0x1p-33f);
-    const float rev = __builtin_bit_cast(float, (wb & 0x007FFFFFu) | 0x3F800000u);
-    const float rr = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1));
-    const float c1 = __builtin_amdgcn_cosf(rev);
+                                                    float& ya, float& va, float u1, float rev,
+                                                    float& lg, float& c1) {
+    lg = __builtin_amdgcn_logf(u1);
+    c1 = __builtin_amdgcn_cosf(rev);
     const float c2 = __builtin_amdgcn_cosf(rev - k.phi);
-    zc = rr * c1;
-    const float t = __builtin_amdgcn_sqrtf(v) * rr;
-    y = __builtin_fmaxf(__builtin_fmaf(t, c1, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
+    const float t = __builtin_amdgcn_sqrtf(-(v * lg));
+    y = floor_max(__builtin_fmaf(t, c1, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
     v = __builtin_fmaxf(__builtin_fmaf(k.sgk * t, c2, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
-    const float ta = __builtin_amdgcn_sqrtf(va) * rr;
-    ya = __builtin_fmaxf(__builtin_fmaf(-ta, c1, ya + __builtin_fmaf(-k.h, va, k.a)), k.yf);
+    const float ta = __builtin_amdgcn_sqrtf(-(va * lg));
+    ya = floor_max(__builtin_fmaf(-ta, c1, ya + __builtin_fmaf(-k.h, va, k.a)), k.yf);
     va = __builtin_fmaxf(__builtin_fmaf(-(k.sgk * ta), c2, __builtin_fmaf(k.omk, va, k.kth)),
                          0.0f);
 }
@@ -452,22 +477,31 @@ __device__ __forceinline__ void heston_px_pair_step(const HestonPx& k, float& y,
 __device__ __forceinline__ void heston_pair_px(uint32_t p, int32_t steps, const HestonPx& k,
                                                uint32_t k0, uint32_t k1, float& xf, float& xa,
                                                float& z1f) {
-    float y = 0.0f, v = k.v0, ya = 0.0f, va = k.v0, zc = 0.0f, z0 = 0.0f;
-    const int32_t nb = steps >> 1;
+    float y = 0.0f, v = k.v0, ya = 0.0f, va = k.v0, lg = 0.0f, c1 = 0.0f, lg0 = 0.0f, c10 = 0.0f;
+    const int32_t nb = steps / 3, rem = steps - 3 * nb;
     for (int32_t b = 0; b < nb; ++b) {
         const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
-        heston_px_pair_step(k, y, v, ya, va, w.x, w.y, zc);
-        if (b == 0) z0 = zc;
-        heston_px_pair_step(k, y, v, ya, va, w.z, w.w, zc);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.x), rev23(w.w), lg, c1);
+        if (b == 0) {
+            lg0 = lg;
+            c10 = c1;
+        }
+        heston_px_pair_step(k, y, v, ya, va, u23(w.y), rev23(top_bytes(w.x, w.y)), lg, c1);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.z), rev23(top_bytes(w.z, w.w)), lg, c1);
     }
-    if (steps & 1) {
+    if (rem) {
         const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
-        heston_px_pair_step(k, y, v, ya, va, w.x, w.y, zc);
-        if (nb == 0) z0 = zc;
+        heston_px_pair_step(k, y, v, ya, va, u23(w.x), rev23(w.w), lg, c1);
+        if (nb == 0) {
+            lg0 = lg;
+            c10 = c1;
+        }
+        if (rem > 1)
+            heston_px_pair_step(k, y, v, ya, va, u23(w.y), rev23(top_bytes(w.x, w.y)), lg, c1);
     }
     xf = y * k.ck;
     xa = ya * k.ck;
-    z1f = z0 * 1.1774100225154746910f;  // sqrt(2 ln 2)
+    z1f = __builtin_amdgcn_sqrtf(-lg0) * c10 * 1.1774100225154746910f;  // sqrt(2 ln 2)
 }
 
 // Heston constants of a record in both forms.

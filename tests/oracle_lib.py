@@ -23,6 +23,7 @@ _ORC = {
     "orc_box_muller": (None, [ct.c_uint32, ct.c_uint32, PD, PD]),
     "orc_philox_path_normal": (D, [U, ct.c_uint32, ct.c_uint32]),
     "orc_philox_terminal_normal": (D, [U, ct.c_uint32]),
+    "orc_philox_heston_normals": (None, [U, ct.c_uint32, ct.c_uint32, PD, PD]),
     "orc_px_terminal": (D, [D] * 6 + [I, I, U, I, PD]),
     "orc_px_stepwise": (D, [I] + [D] * 6 + [I, I, I, U, PD]),
     "orc_px_heston": (D, [D] * 10 + [I, I, I, U, PD]),
@@ -68,6 +69,11 @@ class Oracle:
         o = (ct.c_uint32 * 4)()
         self.lib.orc_philox4x32_10(a, b, o)
         return list(o)
+
+    def heston_normals(self, seed, path, step):
+        a, b = ct.c_double(), ct.c_double()
+        self.lib.orc_philox_heston_normals(seed, path, step, ct.byref(a), ct.byref(b))
+        return a.value, b.value
 
     def px(self, fn, *args):
         """Call an orc_px_* function; returns (price, stderr)."""

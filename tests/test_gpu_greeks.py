@@ -182,12 +182,13 @@ def test_philox_matches_fp64_oracle(qk, orc):
     # Heston LR delta: FP32 pairs driven by one Box-Muller evaluation
     h = wl.config1(8, paths=2001, steps=33)
     h["option_type"] = (np.arange(8) % 2).astype(np.int32)
+    h["steps"] = np.array([33, 34, 35, 1, 2, 3, 4, 64], np.int32)  # every remainder mod 3
     clip = np.full(8, 6.0)
     got, se = qk.heston_lr_delta_batch(*[h[k] for k in HST], h["option_type"], h["paths"], h["steps"],
                                        h["seed"], clip, return_stderr=True)
     for i in range(8):
-        w, wse = orc.px("heston_lr_delta", *_row(h, i, HST), int(h["option_type"][i]), 2001, 33,
-                        int(h["seed"][i]), 6.0)
+        w, wse = orc.px("heston_lr_delta", *_row(h, i, HST), int(h["option_type"][i]), 2001,
+                        int(h["steps"][i]), int(h["seed"][i]), 6.0)
         print("heston_lr", i, got[i], w, se[i], wse)
         assert abs(got[i] - w) < 2e-4, (i, got[i], w)
         assert abs(se[i] - wse) <= 2e-3 * wse + 1e-9

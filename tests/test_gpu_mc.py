@@ -116,11 +116,13 @@ def test_philox_heston_matches_fp64_oracle(qk, orc):
     d = wl.config1(8, paths=3000, steps=33)
     keys = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
     d["option_type"] = (np.arange(8) % 2).astype(np.int32)
+    # every step count mod 3 (three steps per Philox block), and 1-2 steps
+    d["steps"] = np.array([33, 34, 35, 1, 2, 3, 64, 256], np.int32)
     got, se = qk.heston_monte_carlo_price_batch(*[d[k] for k in keys], d["option_type"], d["paths"],
                                                 d["steps"], d["seed"], return_stderr=True)
     for i in range(8):
-        w, wse = orc.px("heston", *[float(d[k][i]) for k in keys], int(d["option_type"][i]), 3000, 33,
-                        int(d["seed"][i]))
+        w, wse = orc.px("heston", *[float(d[k][i]) for k in keys], int(d["option_type"][i]), 3000,
+                        int(d["steps"][i]), int(d["seed"][i]))
         print(i, got[i], w, se[i], wse)
         # FP32 path error per unit spot, and the SE estimate itself
         assert abs(got[i] - w) / d["spot"][i] < 2e-5

@@ -131,6 +131,52 @@ def test_box_muller_exact_mapping(orc):
     assert z0.value == pytest.approx(math.sqrt(2 * 33 * math.log(2)), rel=1e-14) and z1.value == 0.0
 
 
+def test_heston_stream_exact_mapping(orc):
+    """Step j of path p: slot j%3 of Philox block (p, j/3, 0, 0); u1 from the
+    low 23 bits of word j%3 as (2m+1) 2^-24; angle bits disjoint from them.
+    Restated here from the words with numpy and compared for many draws."""
+    seed = 0x1234_5678_9ABC_DEF0
+    for p in (0, 1, 77, 2**31 + 5):
+        for j in range(9):
+            w = orc.philox([p, j // 3, 0, 0], [seed & 0xFFFFFFFF, seed >> 32])
+            s = j % 3
+            u1 = (2.0 * (w[s] & 0x7FFFFF) + 1.0) / 2.0**24
+            if s == 0:
+                fb = w[3] & 0x7FFFFF
+            else:
+                hi, lo = (w[0], w[1]) if s == 1 else (w[2], w[3])
+                fb = ((hi >> 24) & 0x7F) << 16 | (lo >> 24) << 8
+            rad = math.sqrt(-2.0 * math.log(u1))
+            want = (rad * math.cos(2 * math.pi * fb / 2.0**23), rad * math.sin(2 * math.pi * fb / 2.0**23))
+            got = orc.heston_normals(seed, p, j)
+            assert got == pytest.approx(want, rel=1e-13, abs=1e-13), (p, j)
+    # the largest radius: m = 0 -> u1 = 2^-24, sqrt(48 ln 2) = 5.768
+    assert math.sqrt(-2 * math.log(2.0**-24)) == pytest.approx(math.sqrt(48 * math.log(2)), rel=1e-15)
+
+
+def test_heston_stream_distribution(orc):
+    n_paths, n_steps = 4000, 12
+    z = np.array([orc.heston_normals(99, p, j) for p in range(n_paths) for j in range(n_steps)])
+    n = z.size
+    for zz in (z[:, 0], z[:, 1]):
+        assert abs(zz.mean()) < 5 / math.sqrt(zz.size)
+        assert abs(zz.var() - 1) < 5 * math.sqrt(2 / zz.size)
+        k = ((zz - zz.mean()) ** 4).mean() / zz.var() ** 2
+        assert abs(k - 3) < 0.15
+        zs = np.sort(zz)
+        cdf = 0.5 * np.array([math.erfc(-x / math.sqrt(2)) for x in zs])
+        ks = np.max(np.abs(cdf - np.arange(1, zs.size + 1) / zs.size))
+        assert ks < 1.63 / math.sqrt(zs.size) * 1.5
+    # z1 and z2 uncorrelated; no correlation between the three slots of a
+    # block (they share words only through disjoint bits) or across blocks
+    assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 5 / math.sqrt(n)
+    zz = z[:, 0].reshape(n_paths, n_steps)
+    for a, b in ((0, 1), (1, 2), (0, 2), (2, 3), (0, 3)):
+        assert abs(np.corrcoef(zz[:, a], zz[:, b])[0, 1]) < 5 / math.sqrt(n_paths)
+        assert abs(np.corrcoef(zz[:, a] ** 2, zz[:, b] ** 2)[0, 1]) < 5 / math.sqrt(n_paths)
+    assert np.abs(z).max() <= math.sqrt(48 * math.log(2)) + 1e-12
+
+
 def test_bsm_known_values_and_parity(orc):
     c = orc.lib.orc_bsm(100, 100, 1, 0.2, 0.05, 0.0, 0)
     p = orc.lib.orc_bsm(100, 100, 1, 0.2, 0.05, 0.0, 1)
