@@ -47,7 +47,11 @@ FLOPS_PER_PATH_STEP = 37      # Heston step incl. two Box-Muller normals (algori
 MC_PATHS = 65536
 SPOT_EVERY, SPOT_PATHS = 128, 8192
 STEPS = 256
-METRIC = "Heston prices/sec at <=1e-3 rel err: surrogate+fallback vs plain MC, 1 GPU"
+METRIC = "Heston prices/sec at \u22641e-3 rel err: surrogate+fallback vs plain MC, 1 GPU"
+# HBM bytes per launch of each workload's dominant kernel, from rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel of known size
+# (written by the profiling run; absent -> traffic null)
+PMC_TRAFFIC = ROOT / "profiles" / "pmc_traffic.json"
 KEYS = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
 
 
@@ -101,7 +105,24 @@ def cpu_baseline(d, budget_s=12.0):
 
 def to_device(d, dev):
     import torch
-    return {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in d.items()}
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in d.items()}
+    torch.cuda.synchronize()  # the library's stream does not wait for torch's
+    return t
+
+
+def direct_kernel_ms(qk, call, reps=1):
+    """Simulate-kernel time from the HIP events the library records around the
+    kernel.  A replayed launch graph is timed only as a whole, so these
+    batches run with direct launches (same kernels, same shapes)."""
+    qk.graphs(0)
+    try:
+        sims = []
+        for _ in range(reps):
+            call()
+            sims.append(qk.last_timing()[1])
+    finally:
+        qk.graphs(1)
+    return float(np.mean(sims))
 
 
 def plain_mc_rate(qk, qka, d, n, dev, reps=2):
@@ -115,16 +136,17 @@ def plain_mc_rate(qk, qka, d, n, dev, reps=2):
     ptrs = {k: v.data_ptr() for k, v in t.items()}
     out = torch.empty(n, dtype=torch.float64, device=dev)
     se = torch.empty(n, dtype=torch.float64, device=dev)
-    qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+    def call():
+        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+
+    call()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sims = []
     for _ in range(reps):
-        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
-        sims.append(qk.last_timing()[1])
+        call()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / reps
-    return n / wall, float(np.mean(sims)), out.cpu().numpy(), se.cpu().numpy()
+    return n / wall, direct_kernel_ms(qk, call), out.cpu().numpy(), se.cpu().numpy()
 
 
 def bench_c1(args, qk, qka, dev, name, cus):
@@ -138,18 +160,20 @@ def bench_c1(args, qk, qka, dev, name, cus):
     ptrs = {k: v.data_ptr() for k, v in t.items()}
     out = torch.empty(n, dtype=torch.float64, device=dev)
     se = torch.empty(n, dtype=torch.float64, device=dev)
-    for _ in range(args.warmup):
+    def call():
         qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+
+    for _ in range(args.warmup):
+        call()
     torch.cuda.synchronize()
-    sims = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
-        sims.append(qk.last_timing()[1])
-        log(f"step done: sim kernel {sims[-1]:.2f} ms")
+        call()  # a replayed launch graph
+        log(f"step done: batch {qk.last_timing()[0]:.2f} ms")
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    sim_ms = float(np.mean(sims))
+    sim_ms = direct_kernel_ms(qk, call)
+    log(f"simulate kernel {sim_ms:.2f} ms")
     flops = float(n) * MC_PATHS * STEPS * FLOPS_PER_PATH_STEP
     ach = flops / (sim_ms * 1e-3) / 1e12
     res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
@@ -198,27 +222,46 @@ def cpu_sample(fn, arrays, n, threads, budget_s=10.0, per=1):
 
 def bench_c0(args, qk, qka, dev, name, cus):
     from quantkernel_amd import workloads as wl
+    import torch
     n = args.options or 4096
     d = wl.config0(n)
     a = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths")]
+    t = to_device(d, dev)
+    ptrs = {k: v.data_ptr() for k, v in t.items()}
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    seo = torch.empty(n, dtype=torch.float64, device=dev)
+
+    def call():
+        qk.standard_monte_carlo_price_batch_device(n, ptrs, out.data_ptr(), seo.data_ptr())
+
     for _ in range(args.warmup):
-        qk.standard_monte_carlo_price_batch(*a, d["seed"])
-    tot, sims = [], []
+        call()
+    torch.cuda.synchronize()
+    tot = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        p, se = qk.standard_monte_carlo_price_batch(*a, d["seed"], return_stderr=True)
-        tt, sm = qk.last_timing()
-        tot.append(tt)
-        sims.append(sm)
-        log(f"step done: device {tt:.3f} ms, terminal kernel {sm:.3f} ms")
+        call()  # a replayed launch graph
+        tot.append(qk.last_timing()[0])
+        log(f"step done: device {tot[-1]:.3f} ms")
+    torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    sim_ms = float(np.mean(sims))
+    sim_ms = direct_kernel_ms(qk, call, reps=5)
+    se = seo.cpu().numpy()
+    # the host-memory entry point on the same batch (adds the PCIe copies)
+    reps = max(args.steps, 5)
+    qk.standard_monte_carlo_price_batch(*a, d["seed"])
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        qk.standard_monte_carlo_price_batch(*a, d["seed"])
+    host_rate = n * reps / (time.perf_counter() - t1)
     flops = float(n) * 100000 * 11
     ach = flops / (sim_ms * 1e-3) / 1e12
     res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
                config={"workload": f"C0: {n} GBM European calls/puts, standard MC (antithetic terminal), 100000 paths "
-                                   "(Philox, FP32 paths, FP64 sums); host-memory entry point", "options": n,
+                                   "(Philox, FP32 paths, FP64 sums); device-resident entry point, one launch graph "
+                                   "per batch", "options": n,
                        "device_ms": float(np.mean(tot)), "device": name, "cus": cus,
+                       "host_entry_prices_per_s_incl_pcie": host_rate,
                        "median_stderr_over_spot": float(np.median(se / d["spot"]))},
                roofline={"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
@@ -242,18 +285,24 @@ def bench_c2(args, qk, qka, dev, name, cus):
     ptrs = {k: v.data_ptr() for k, v in t.items()}
     out = torch.empty(n, dtype=torch.float64, device=dev)
     se = torch.empty(n, dtype=torch.float64, device=dev)
-    for _ in range(args.warmup):
+    # tile shape from the autotuner (timed on C2-like synthetic batches)
+    lane, grid, tune_ms = qk.autotune(qk.FAMILY_STEPWISE)
+    log(f"autotuned stepwise shape: {lane} paths/lane, {grid} workgroups/CU")
+
+    def call():
         qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
+
+    for _ in range(args.warmup):
+        call()
     torch.cuda.synchronize()
-    sims = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        qk.mixed_price_batch_device(n, ptrs, out.data_ptr(), se.data_ptr())
-        sims.append(qk.last_timing()[1])
-        log(f"step done: sim kernel {sims[-1]:.2f} ms")
+        call()  # a replayed launch graph
+        log(f"step done: batch {qk.last_timing()[0]:.2f} ms")
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    sim_ms = float(np.mean(sims))
+    sim_ms = direct_kernel_ms(qk, call)
+    log(f"simulate kernel {sim_ms:.2f} ms")
     per_step = {0: 14, 1: 18, 2: FLOPS_PER_PATH_STEP, 3: 15}
     ps = d["paths"].astype(np.float64) * d["steps"]
     flops = float(sum(ps[d["model"] == m].sum() * f for m, f in per_step.items()))
@@ -262,8 +311,10 @@ def bench_c2(args, qk, qka, dev, name, cus):
     res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
                config={"workload": f"C2: {n} uneven options, 25% each GBM-Euler / GBM-Milstein / Heston-Euler / "
                                    "constant local vol; paths log-uniform [4096, 1048576], steps log-uniform "
-                                   "[16, 512]; one persistent launch (device-resident entry point)",
+                                   "[16, 512]; one persistent launch, replayed as a launch graph (device-resident "
+                                   "entry point)",
                        "options": n, "path_steps": float(ps.sum()), "device": name, "cus": cus,
+                       "autotuned_shape": {"paths_per_lane": lane, "workgroups_per_cu": grid},
                        "median_stderr_over_spot": float(np.median(sev)), "max_stderr_over_spot": float(np.max(sev))},
                roofline={"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
@@ -310,10 +361,12 @@ def bench_c3(args, qk, qka, dev, name, cus):
         log(f"step done: device {times[-1]:.2f} ms")
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     dev_ms = float(np.mean(times))
-    # algorithmic HBM bytes: write S (4 B) once, then per regression date read S_j (4 B)
-    # and read+write the FP32 cashflow (8 B): 4 + 12 * (dates - 1) bytes per path
+    # algorithmic HBM bytes per path: the forward pass writes S at every date
+    # (4 B x dates); each of the dates - 1 regression dates then reads S twice
+    # (its regression sums, and the exercise rule once the fit is known) and
+    # reads and writes the FP32 cashflow: 16 B per path-date
     paths, dates = 100000, 50
-    byts = float(n) * paths * (4 * dates + 12 * (dates - 1))
+    byts = float(n) * paths * (4 * dates + 16 * (dates - 1))
     ach = byts / (dev_ms * 1e-3) / 1e9
     res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
                config={"workload": f"C3: {n} American puts, Longstaff-Schwartz, {paths} paths x {dates} dates, basis "
@@ -321,7 +374,9 @@ def bench_c3(args, qk, qka, dev, name, cus):
                        "options": n, "device": name, "cus": cus,
                        "median_stderr_over_spot": float(np.median(se / d["spot"]))},
                roofline={"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
-                         "traffic": None, "kernel": "qk::k_lsmc<float, PHILOX, 4>", "kernel_ms": dev_ms,
+                         "traffic": None,
+                         "kernel": "LSMC pipeline: k_lsmc_forward, 49 x (k_lsmc_solve, k_lsmc_sweep), k_lsmc_final",
+                         "kernel_ms": dev_ms,
                          "bytes_per_launch": byts})
     if not args.no_cpu:
         kind, fn = ref_fn("qk_mcm_longstaff_schwartz_price_batch")
@@ -435,12 +490,26 @@ def main():
     ap.add_argument("--weights", default=str(ROOT / "quant-kernel_amd" / "quantkernel_amd" / "heston_surrogate.qkw"))
     ap.add_argument("--accuracy-queries", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
+                    help="only run one elementwise copy of MIB MiB (y = x + 1, fp32) on the GPU: the "
+                         "known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(options shard independently across GPUs; see the design notes).", file=sys.stderr)
+              "(options are independent and would shard across GPUs by option; "
+              "the value reported is one GPU's throughput).", file=sys.stderr)
         sys.exit(2)
     import torch
+
+    if args.calibration_copy > 0:
+        n = args.calibration_copy << 18  # fp32 elements
+        x = torch.zeros(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        torch.add(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes_read": n * 4, "calibration_copy_bytes_written": n * 4}))
+        return
 
     import quantkernel_amd as qka
 
@@ -450,6 +519,11 @@ def main():
     workload = args.workload or ("c4" if Path(args.weights).exists() else "c1")
     res = {"c0": bench_c0, "c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[workload](
         args, qk, qka, dev, name, cus)
+    if PMC_TRAFFIC.exists():
+        entry = json.loads(PMC_TRAFFIC.read_text()).get(workload)
+        if entry and entry.get("kernel") == res["roofline"]["kernel"]:
+            res["roofline"]["traffic"] = entry["traffic_bytes"]
+            res["roofline"]["traffic_source"] = f"rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, {PMC_TRAFFIC.name}"
     line = {"metric": METRIC, "value": res["value"], "unit": "prices/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": res["dtype"], "data": "synthetic", "config": res["config"],

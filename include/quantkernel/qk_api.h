@@ -253,13 +253,27 @@ QK_EXPORT int32_t qk_mcm_mixed_price_batch(const int32_t* model,
 
 /* Same as qk_mcm_mixed_price_batch, but every pointer is device memory of
  * the current HIP device (inputs already resident in HBM, outputs left there).
- * QK_RNG_PHILOX only; returns QK_ERR_INVALID_INPUT in another RNG mode. */
+ * QK_RNG_PHILOX only; returns QK_ERR_INVALID_INPUT in another RNG mode.  The
+ * library works on its own stream, which does not wait for the caller's
+ * streams: inputs must be complete when the call is made.  The call returns
+ * when the outputs are written. */
 QK_EXPORT int32_t qk_mcm_mixed_price_batch_device(const int32_t* model,
         const double* spot, const double* strike, const double* t, const double* vol,
         const double* r, const double* q,
         const double* v0, const double* kappa, const double* theta, const double* sigma,
         const double* rho, const int32_t* option_type, const int32_t* paths, const int32_t* steps,
         const uint64_t* seed, int32_t n, double* out_prices, double* out_stderr);
+
+/* Standard MC (antithetic terminal estimator) on device-resident inputs and
+ * outputs, with standard errors; QK_RNG_PHILOX only.  Equal to
+ * qk_mcm_standard_monte_carlo_price_batch_se for the same inputs.
+ * Device-resident calls run without a host synchronisation inside the batch,
+ * so each distinct (size, buffers) shape is captured once as a launch graph
+ * and replayed (see qk_mc_graphs). */
+QK_EXPORT int32_t qk_mcm_standard_monte_carlo_price_batch_device(const double* spot,
+        const double* strike, const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const uint64_t* seed, int32_t n,
+        double* out_prices, double* out_stderr);
 
 /* Longstaff-Schwartz with a selectable regression basis (QK_LSMC_BASIS_*)
  * and standard-error output.  basis QK_LSMC_BASIS_REFERENCE reproduces
@@ -271,8 +285,10 @@ QK_EXPORT int32_t qk_mcm_longstaff_schwartz_price_batch_ex(const double* spot, c
 
 /* Device timings of the last MC call made by this process, from HIP events on
  * the library's stream: total device time of the call and the time of its
- * path-simulation kernel alone, in milliseconds.  Returns QK_ERR_RUNTIME_INIT
- * if no call has completed yet. */
+ * path-simulation kernel alone, in milliseconds.  A host-memory QK_RNG_PHILOX
+ * call that ran as a launch graph (qk_mc_graphs) is timed as a whole (copies
+ * included) and reports ms_simulate = NaN.  Returns QK_ERR_RUNTIME_INIT if no
+ * call has completed yet. */
 QK_EXPORT int32_t qk_mc_last_timing(double* ms_total, double* ms_simulate);
 
 /* The normal stream the QK_RNG_MT19937_* modes feed to the GPU for one
@@ -287,9 +303,10 @@ QK_EXPORT int32_t qk_mc_validation_stream(uint64_t seed, int64_t count, double* 
 QK_EXPORT int32_t qk_mc_philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3,
                                      int32_t count, uint32_t* out_words, float* out_normals);
 
-/* Launch graphs of the host-memory QK_RNG_PHILOX path (on by default): each
- * distinct batch shape is captured once (H2D copy, prepare, scan, simulate,
- * finalize, D2H copy) and replayed.  enable: 1 on, 0 off, -1 unchanged.
+/* Launch graphs of the QK_RNG_PHILOX paths (on by default): each distinct
+ * batch shape of the host-memory entry points is captured once (H2D copy,
+ * prepare, scan, simulate, finalize, D2H copy) and replayed, and likewise each
+ * (size, buffers) shape of the device-resident entry points.  enable: 1 on, 0 off, -1 unchanged.
  * out_cached: graphs held (-1 if the runtime cannot capture); out_launches:
  * graph launches so far.  Either output may be null.  QK_NO_GRAPH=1 turns
  * graphs off for the process. */

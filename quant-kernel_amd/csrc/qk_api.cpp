@@ -571,6 +571,35 @@ int32_t qk_mcm_mixed_price_batch_device(const int32_t* model, const double* spot
     return rc;
 }
 
+int32_t qk_mcm_standard_monte_carlo_price_batch_device(const double* spot, const double* strike,
+                                                       const double* t, const double* vol,
+                                                       const double* r, const double* q,
+                                                       const int32_t* option_type,
+                                                       const int32_t* paths, const uint64_t* seed,
+                                                       int32_t n, double* out_prices,
+                                                       double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    qk::BatchIn bi{};
+    bi.uniform_method = qk::M_STANDARD;
+    bi.spot = spot;
+    bi.strike = strike;
+    bi.t = t;
+    bi.vol = vol;
+    bi.r = r;
+    bi.q = q;
+    bi.ot = option_type;
+    bi.paths = paths;
+    bi.seed = seed;
+    std::string err;
+    const int32_t rc = qk::run_mc_batch_device(bi, n, out_prices, out_stderr, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
 // ---- RNG mode, timing, device info (ABI 2.13) -----------------------------
 int32_t qk_mc_set_rng_mode(int32_t mode) {
     if (mode != QK_RNG_PHILOX && mode != QK_RNG_MT19937_FP64 && mode != QK_RNG_MT19937_FP32) {

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
@@ -115,22 +116,22 @@ size_t align64(size_t x) { return (x + 63) & ~size_t(63); }
 int32_t launch_pipeline(Device* d, const BatchIn& in, int32_t n, const int32_t* n_dev,
                         const int64_t* stream_off, int32_t family, int32_t mode,
                         const void* stream, double* out_price, double* out_se,
-                        std::string* err) {
+                        std::string* err, bool timed = true, int32_t capacity = 0x7fffffff) {
     OptionRec* recs = static_cast<OptionRec*>(d->recs.p);
     int32_t* ntiles = static_cast<int32_t*>(d->ntiles.p);
     int32_t* tstart = static_cast<int32_t*>(d->tile_start.p);
     int32_t* ctr = static_cast<int32_t*>(d->counters.p);
     Partial* parts = static_cast<Partial*>(d->partials.p);
     const SimShape& shp = family == FAM_TERMINAL ? d->shape_term : d->shape_step;
-    QK_HIP(hipEventRecord(d->ev[0], d->stream));
+    if (timed) QK_HIP(hipEventRecord(d->ev[0], d->stream));
     QK_HIP(launch_prepare(d->stream, in, n_dev, stream_off, tile_units_terminal(d->shape_term),
                           tile_units_stepwise(d->shape_step), recs, ntiles));
-    QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p));
-    QK_HIP(hipEventRecord(d->ev[1], d->stream));
+    QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p, capacity));
+    if (timed) QK_HIP(hipEventRecord(d->ev[1], d->stream));
     QK_HIP(launch_simulate(d->stream, family, mode, shp, recs, tstart, n_dev, ctr, parts, stream));
-    QK_HIP(hipEventRecord(d->ev[2], d->stream));
-    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, out_price, out_se));
-    QK_HIP(hipEventRecord(d->ev[3], d->stream));
+    if (timed) QK_HIP(hipEventRecord(d->ev[2], d->stream));
+    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, ctr, out_price, out_se));
+    if (timed) QK_HIP(hipEventRecord(d->ev[3], d->stream));
     return QK_OK;
 }
 
@@ -140,6 +141,16 @@ void record_timing(Device* d) {
         hipEventElapsedTime(&b, d->ev[1], d->ev[2]) == hipSuccess) {
         d->ms_total = a;
         d->ms_sim = b;
+    }
+}
+
+// A replayed launch graph is timed as a whole (events recorded around it on
+// the stream); its simulate kernel is not timed separately.
+void record_timing_total(Device* d) {
+    float a = 0.f;
+    if (hipEventElapsedTime(&a, d->ev[0], d->ev[3]) == hipSuccess) {
+        d->ms_total = a;
+        d->ms_sim = std::nan("");
     }
 }
 
@@ -293,19 +304,18 @@ int32_t stage_host_batch(Device* d, const HostBatch& hb, int32_t n, BatchIn* bi_
 
 namespace {
 
-template <class F>
-int32_t enqueue_captured(F& enqueue, std::string*) {
-    return enqueue();
-}
-
 // Runs `enqueue` on d->stream through a cached instantiated graph when
 // possible.  The first call for a key captures and instantiates; later calls
 // with the same key only launch.  Any capture or instantiation failure turns
 // graphs off for the process and runs the work directly.
+bool graphs_active(const Device* d) {
+    static const bool disabled = env_int("QK_NO_GRAPH", 0) != 0;
+    return !disabled && d->graphs_ok && d->graphs_enabled;
+}
+
 template <class F>
 int32_t run_graphed(Device* d, const std::vector<uint64_t>& key, F&& enqueue, std::string* err) {
-    static const bool disabled = env_int("QK_NO_GRAPH", 0) != 0;
-    if (disabled || !d->graphs_ok || !d->graphs_enabled) return enqueue();
+    if (!graphs_active(d)) return enqueue();
     for (auto& g : d->graphs)
         if (g.key == key) {
             QK_HIP(hipGraphLaunch(g.exec, d->stream));
@@ -317,8 +327,7 @@ int32_t run_graphed(Device* d, const std::vector<uint64_t>& key, F&& enqueue, st
         d->graphs_ok = false;
         return enqueue();
     }
-    std::string cap_err;
-    const int32_t rc = enqueue_captured(enqueue, &cap_err);
+    const int32_t rc = enqueue();
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(d->stream, &graph);
     hipGraphExec_t exec = nullptr;
@@ -441,11 +450,13 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
         d_stream = d->stream_buf.p;
     }
 
+    const bool use_graph = graphed && graphs_active(d);
     auto enqueue = [&]() -> int32_t {
         if (graphed)
             QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, in_copy, hipMemcpyHostToDevice, d->stream));
+        // inside a graph the pipeline records no events (they are taken around it)
         int32_t rc = launch_pipeline(d, bi, n, n_dev, d_off, family, sim_mode, d_stream, dprice, dse,
-                                     err);
+                                     err, !use_graph);
         if (rc != QK_OK) return rc;
         // outputs + the scan's overflow flag
         QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, ns * 16, hipMemcpyDeviceToHost, d->stream));
@@ -454,7 +465,7 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
         return QK_OK;
     };
     int32_t rc;
-    if (graphed) {
+    if (use_graph) {
         const SimShape& st = d->shape_step;
         const SimShape& tt = d->shape_term;
         uint64_t present = 0;
@@ -463,7 +474,7 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
                                 bi.seed, bi.aux};
         for (int k = 0; k < 17; ++k) present |= uint64_t(fields[k] != nullptr) << k;
         const std::vector<uint64_t> key = {
-            uint64_t(n), uint64_t(family), uint64_t(bi.uniform_method), uint64_t(bi.method_is_model),
+            1, uint64_t(n), uint64_t(family), uint64_t(bi.uniform_method), uint64_t(bi.method_is_model),
             present, uint64_t(st.lane_units), uint64_t(st.grid), uint64_t(tt.lane_units),
             uint64_t(tt.grid), uint64_t(in_copy), uint64_t(uintptr_t(d->in.p)),
             uint64_t(uintptr_t(d->pin_in.p)), uint64_t(uintptr_t(d->recs.p)),
@@ -471,13 +482,16 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
             uint64_t(uintptr_t(d->counters.p)), uint64_t(uintptr_t(d->scan.p)),
             uint64_t(uintptr_t(d->partials.p)), uint64_t(uintptr_t(d->out.p)),
             uint64_t(uintptr_t(d->pin_out.p))};
+        QK_HIP(hipEventRecord(d->ev[0], d->stream));
         rc = run_graphed(d, key, enqueue, err);
+        if (rc == QK_OK) QK_HIP(hipEventRecord(d->ev[3], d->stream));
     } else {
         rc = enqueue();
     }
     if (rc != QK_OK) return rc;
     QK_HIP(hipStreamSynchronize(d->stream));
-    record_timing(d);
+    if (use_graph) record_timing_total(d);
+    else record_timing(d);
     const int32_t* hc = reinterpret_cast<const int32_t*>(static_cast<char*>(d->pin_out.p) + ns * 16);
     if (hc[2]) {
         if (err) *err = "batch too large: more than 2^31-1 path tiles";
@@ -500,7 +514,7 @@ int32_t mc_device_locked(Device* d, const BatchIn& in, int32_t n, const int32_t*
     QK_HIP(hipEventRecord(d->ev[0], d->stream));
     QK_HIP(launch_prepare(d->stream, in, n_dev, nullptr, tile_units_terminal(d->shape_term),
                           tile_units_stepwise(d->shape_step), recs, ntiles));
-    QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p));
+    QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p, 0x7fffffff));
     QK_HIP(hipMemcpyAsync(d->pin_out.p, ctr, 16, hipMemcpyDeviceToHost, d->stream));
     QK_HIP(hipStreamSynchronize(d->stream));
     const int32_t* hc = static_cast<const int32_t*>(d->pin_out.p);
@@ -517,7 +531,7 @@ int32_t mc_device_locked(Device* d, const BatchIn& in, int32_t n, const int32_t*
     QK_HIP(launch_simulate(d->stream, family, SIM_PHILOX_F32, shp, recs, tstart, n_dev, ctr, parts,
                            nullptr));
     QK_HIP(hipEventRecord(d->ev[2], d->stream));
-    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, out_price_dev, out_se_dev));
+    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, ctr, out_price_dev, out_se_dev));
     QK_HIP(hipEventRecord(d->ev[3], d->stream));
     return QK_OK;
 }
@@ -547,14 +561,66 @@ int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
     if (rc != QK_OK) return rc;
     QK_HIP(d->pin_in.ensure(64));
     QK_HIP(d->in.ensure(64));
-    std::memcpy(d->pin_in.p, &n, 4);
-    QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, 4, hipMemcpyHostToDevice, d->stream));
-    rc = mc_device_locked(d, in, n, static_cast<const int32_t*>(d->in.p), out_price_dev, out_se_dev,
-                          err);
-    if (rc != QK_OK) return rc;
-    QK_HIP(hipStreamSynchronize(d->stream));
-    record_timing(d);
-    return QK_OK;
+    // The tile count is only known on the device, so the partials buffer has a
+    // capacity (grow-only, at least QK_DEVICE_TILES tiles); a batch that needs
+    // more runs no tile, and is run again once after the buffer has grown.
+    // Without a mid-batch synchronisation the whole pipeline is one graph.
+    const int64_t min_tiles = std::max(1, env_int("QK_DEVICE_TILES", 1 << 20));
+    QK_HIP(d->partials.ensure(size_t(min_tiles) * sizeof(Partial)));
+    const int32_t family =
+        (!in.method && is_terminal_method(in.uniform_method)) ? FAM_TERMINAL : FAM_STEPWISE;
+    const int32_t* n_dev = static_cast<const int32_t*>(d->in.p);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const int32_t cap =
+            int32_t(std::min<size_t>(d->partials.cap / sizeof(Partial), size_t(0x7fffffff)));
+        std::memcpy(d->pin_in.p, &n, 4);
+        const bool use_graph = attempt == 0 && graphs_active(d);
+        auto enqueue = [&]() -> int32_t {
+            QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, 4, hipMemcpyHostToDevice, d->stream));
+            int32_t r = launch_pipeline(d, in, n, n_dev, nullptr, family, SIM_PHILOX_F32, nullptr,
+                                        out_price_dev, out_se_dev, err, !use_graph, cap);
+            if (r != QK_OK) return r;
+            QK_HIP(hipMemcpyAsync(d->pin_out.p, d->counters.p, 16, hipMemcpyDeviceToHost,
+                                  d->stream));
+            return QK_OK;
+        };
+        if (use_graph) {
+            const SimShape& st = d->shape_step;
+            const SimShape& tt = d->shape_term;
+            const void* fields[] = {in.method, in.spot, in.strike, in.t, in.vol, in.r, in.q,
+                                    in.v0, in.kappa, in.theta, in.sigma, in.rho, in.ot,
+                                    in.paths, in.steps, in.seed, in.aux, in.active,
+                                    out_price_dev, out_se_dev};
+            std::vector<uint64_t> key = {
+                2, uint64_t(n), uint64_t(family), uint64_t(in.uniform_method),
+                uint64_t(in.method_is_model), uint64_t(uint32_t(in.paths_override)),
+                uint64_t(st.lane_units), uint64_t(st.grid), uint64_t(tt.lane_units),
+                uint64_t(tt.grid), uint64_t(cap), uint64_t(uintptr_t(d->in.p)),
+                uint64_t(uintptr_t(d->pin_in.p)), uint64_t(uintptr_t(d->recs.p)),
+                uint64_t(uintptr_t(d->ntiles.p)), uint64_t(uintptr_t(d->tile_start.p)),
+                uint64_t(uintptr_t(d->counters.p)), uint64_t(uintptr_t(d->scan.p)),
+                uint64_t(uintptr_t(d->partials.p)), uint64_t(uintptr_t(d->pin_out.p))};
+            for (const void* f : fields) key.push_back(uint64_t(uintptr_t(f)));
+            QK_HIP(hipEventRecord(d->ev[0], d->stream));
+            rc = run_graphed(d, key, enqueue, err);
+            if (rc == QK_OK) QK_HIP(hipEventRecord(d->ev[3], d->stream));
+        } else {
+            rc = enqueue();
+        }
+        if (rc != QK_OK) return rc;
+        QK_HIP(hipStreamSynchronize(d->stream));
+        if (use_graph) record_timing_total(d);
+        else record_timing(d);
+        const int32_t* hc = static_cast<const int32_t*>(d->pin_out.p);
+        if (hc[2] == 0) return QK_OK;
+        if (hc[2] == 2) {
+            if (err) *err = "batch too large: more than 2^31-1 path tiles";
+            return QK_ERR_INVALID_INPUT;
+        }
+        QK_HIP(d->partials.ensure(size_t(hc[3]) * sizeof(Partial)));
+    }
+    if (err) *err = "tile partials could not be sized for the batch";
+    return QK_ERR_RUNTIME_INIT;
 }
 
 // Longstaff-Schwartz batch: host validation (longstaff_schwartz.cpp:50-55),

@@ -201,7 +201,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_blocks(const int32_t* __res
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_top(int32_t nblocks,
                                                          long long* __restrict__ block_sum,
-                                                         int32_t* __restrict__ counters) {
+                                                         int32_t* __restrict__ counters,
+                                                         int32_t capacity) {
     __shared__ long long sh[kScanBlock];
     long long carry = 0;
     for (int32_t base = 0; base < nblocks; base += kScanBlock) {
@@ -212,11 +213,15 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_top(int32_t nblocks,
         carry += sh[kScanBlock - 1];
         __syncthreads();
     }
+    // counters: [0] tile counter, [1] tiles to run, [2] 0 ok / 1 more tiles than
+    // the partials hold / 2 more than 2^31-1, [3] tiles needed (clamped)
     if (threadIdx.x == 0) {
-        const bool overflow = carry > 0x7fffffffLL;
+        const bool too_big = carry > 0x7fffffffLL;
+        const bool over = too_big || carry > capacity;
         counters[0] = 0;
-        counters[1] = overflow ? 0 : int32_t(carry);
-        counters[2] = overflow ? 1 : 0;
+        counters[1] = over ? 0 : int32_t(carry);
+        counters[2] = too_big ? 2 : (over ? 1 : 0);
+        counters[3] = too_big ? 0x7fffffff : int32_t(carry);
     }
 }
 
@@ -778,14 +783,21 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
 __global__ __launch_bounds__(256) void k_finalize(const int32_t* __restrict__ n_dev,
                                                   const OptionRec* __restrict__ recs,
                                                   const Partial* __restrict__ partials,
+                                                  const int32_t* __restrict__ counters,
                                                   double* __restrict__ out_price,
                                                   double* __restrict__ out_se) {
     const int32_t n = *n_dev;
+    const bool overflow = counters[2] != 0;  // no tile ran: partials were not written
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const OptionRec& rec = recs[i];
         if (rec.family == FAM_NONE) {
             out_price[i] = rec.closed;
             out_se[i] = rec.closed == rec.closed ? 0.0 : qnan_d();
+            continue;
+        }
+        if (overflow) {
+            out_price[i] = qnan_d();
+            out_se[i] = qnan_d();
             continue;
         }
         // Neumaier compensated sums over the option's tiles, in tile order.
@@ -878,12 +890,13 @@ hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_de
 }
 
 hipError_t launch_scan(hipStream_t st, int32_t n, const int32_t* n_dev, const int32_t* ntiles,
-                       OptionRec* recs, int32_t* tile_start, int32_t* counters, void* scratch) {
+                       OptionRec* recs, int32_t* tile_start, int32_t* counters, void* scratch,
+                       int32_t capacity) {
     // n + 1 elements so that the thread of index n writes tile_start[n]
     const int32_t nb = (n + 1 + kScanBlock - 1) / kScanBlock;
     long long* bs = static_cast<long long*>(scratch);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, st, n_dev, ntiles, bs);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, st, nb, bs, counters);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, st, nb, bs, counters, capacity);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, st, n_dev, ntiles, bs, counters,
                        recs, tile_start);
     return hipGetLastError();
@@ -935,9 +948,10 @@ hipError_t launch_simulate(hipStream_t st, int32_t family, int32_t mode, const S
 }
 
 hipError_t launch_finalize(hipStream_t st, const int32_t* n_dev, const OptionRec* recs,
-                           const Partial* partials, double* out_price, double* out_se) {
-    hipLaunchKernelGGL(k_finalize, dim3(256), dim3(256), 0, st, n_dev, recs, partials, out_price,
-                       out_se);
+                           const Partial* partials, const int32_t* counters, double* out_price,
+                           double* out_se) {
+    hipLaunchKernelGGL(k_finalize, dim3(256), dim3(256), 0, st, n_dev, recs, partials, counters,
+                       out_price, out_se);
     return hipGetLastError();
 }
 

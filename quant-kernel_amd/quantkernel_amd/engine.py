@@ -120,6 +120,7 @@ _SIGNATURES = {
     "qk_agm_likelihood_ratio_delta_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PU, _PD, _I, _PD, _PD]),
     "qk_mcm_mixed_price_batch": (_I, [_PI] + [_PD] * 11 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
     "qk_mcm_mixed_price_batch_device": (_I, [ct.c_void_p] * 16 + [_I, ct.c_void_p, ct.c_void_p]),
+    "qk_mcm_standard_monte_carlo_price_batch_device": (_I, [ct.c_void_p] * 9 + [_I, ct.c_void_p, ct.c_void_p]),
     "qk_mc_set_rng_mode": (_I, [_I]),
     "qk_mc_get_rng_mode": (_I, []),
     "qk_mc_last_timing": (_I, [_PD, _PD]),
@@ -619,6 +620,15 @@ class QuantKernel:
         n = m.shape[0]
         self._check_same_length(n, spot=f[0], option_type=ot, paths=p, steps=st, seed=s)
         return self._call_batch("qk_mcm_mixed_price_batch", m, *f, ot, p, st, s, with_se=True)
+
+    def standard_monte_carlo_price_batch_device(self, n: int, ptrs: dict, out_price_ptr: int,
+                                                out_se_ptr: int) -> None:
+        """Device-resident standard MC: ``ptrs`` maps argument names to device addresses."""
+        order = ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths", "seed")
+        args = [ct.c_void_p(int(ptrs[k])) for k in order]
+        self._check(self._lib.qk_mcm_standard_monte_carlo_price_batch_device(
+            *args, int(n), ct.c_void_p(int(out_price_ptr)), ct.c_void_p(int(out_se_ptr))),
+            "qk_mcm_standard_monte_carlo_price_batch_device")
 
     def mixed_price_batch_device(self, n: int, ptrs: dict, out_price_ptr: int, out_se_ptr: int) -> None:
         """Device-resident variant: ``ptrs`` maps argument names to device addresses."""

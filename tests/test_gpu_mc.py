@@ -344,3 +344,81 @@ def test_autotune_and_shapes(qk):
     assert np.allclose(odd, base, rtol=1e-12, atol=1e-12)
     qk.set_shape(fam, *saved)
     assert np.array_equal(qk.mixed_price_batch(*a)[0], base)
+
+
+def _to_dev(d, keys):
+    from hip_mem import to_device
+    return to_device(d, keys)
+
+
+GBM_KEYS = ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths", "seed")
+
+
+def test_device_entry_points_equal_host_paths_and_replay(qk):
+    """Device-resident batches run as replayed launch graphs; every replay,
+    with other device work in between, equals the host-memory entry point."""
+    from hip_mem import DevArray
+    qk.graphs(1)
+    d = wl.config0(300, paths=20001)
+    t = _to_dev(d, GBM_KEYS)
+    ptrs = {k: v.ptr for k, v in t.items()}
+    out = DevArray(n=300)
+    se = DevArray(n=300)
+    host_p, host_se = qk.standard_monte_carlo_price_batch(*[d[k] for k in GBM_KEYS[:-1]], d["seed"],
+                                                          return_stderr=True)
+    _, before = qk.graphs()
+    for i in range(3):
+        qk.standard_monte_carlo_price_batch_device(300, ptrs, out.ptr, se.ptr)
+        assert np.array_equal(out.numpy(), host_p) and np.array_equal(se.numpy(), host_se), i
+        out.fill_bytes(0xFF)  # eager device work between replays
+        e = wl.config1(5, paths=999, steps=7)
+        qk.heston_monte_carlo_price_batch(*[e[k] for k in ("spot", "strike", "t", "r", "q", "v0", "kappa",
+                                                           "theta", "sigma", "rho")],
+                                          e["option_type"], e["paths"], e["steps"], e["seed"])
+    _, after = qk.graphs()
+    assert after - before >= 6  # 3 device replays + 3 host-path Heston batches
+    # mixed-model device batch == host mixed batch
+    m = wl.config2(200, p_lo=1000, p_hi=20000, s_lo=2, s_hi=30)
+    mk = ("model", "spot", "strike", "t", "vol", "r", "q", "v0", "kappa", "theta", "sigma", "rho",
+          "option_type", "paths", "steps", "seed")
+    hp, hs = qk.mixed_price_batch(*[m[k] for k in mk])
+    tm = _to_dev(m, mk)
+    o2, s2 = DevArray(n=200), DevArray(n=200)
+    for _ in range(2):
+        qk.mixed_price_batch_device(200, {k: v.ptr for k, v in tm.items()}, o2.ptr, s2.ptr)
+        assert np.array_equal(o2.numpy(), hp) and np.array_equal(s2.numpy(), hs)
+        o2.fill_bytes(0)
+
+
+def test_device_partials_grow_beyond_capacity(qk):
+    """A device batch needing more tiles than the partials buffer holds runs
+    again after growing it: 40 options x (2^31 - 1) paths = 5.2M tiles."""
+    from hip_mem import DevArray
+    n = 40
+    d = wl.config0(n, paths=1)
+    d["paths"] = np.full(n, 2**31 - 1, np.int32)
+    t = _to_dev(d, GBM_KEYS)
+    out, se = DevArray(n=n), DevArray(n=n)
+    qk.standard_monte_carlo_price_batch_device(n, {k: v.ptr for k, v in t.items()}, out.ptr, se.ptr)
+    host_p, host_se = qk.standard_monte_carlo_price_batch(*[d[k] for k in GBM_KEYS[:-1]], d["seed"],
+                                                          return_stderr=True)
+    got = out.numpy()
+    assert np.all(np.isfinite(got))
+    assert np.array_equal(got, host_p) and np.array_equal(se.numpy(), host_se)
+
+
+def test_timing_in_graph_and_direct_modes(qk):
+    d = wl.config0(64, paths=5001)
+    args = [d[k] for k in GBM_KEYS[:-1]] + [d["seed"]]
+    qk.graphs(1)
+    qk.standard_monte_carlo_price_batch(*args)
+    qk.standard_monte_carlo_price_batch(*args)
+    total, sim = qk.last_timing()
+    assert total > 0 and math.isnan(sim)  # a replayed graph is timed as a whole
+    qk.graphs(0)
+    try:
+        qk.standard_monte_carlo_price_batch(*args)
+        total, sim = qk.last_timing()
+        assert total > 0 and 0 < sim <= total
+    finally:
+        qk.graphs(1)
