@@ -126,6 +126,54 @@ QK_EXPORT int32_t qk_mcm_local_vol_monte_carlo_price_batch(const double* spot, c
                                                            const uint64_t* seed,
                                                            int32_t n, double* out_prices);
 
+/* ---- control variates, stratified sampling, quasi-MC, multilevel MC
+ *      (reference qk_api.h:278-299, 318-358) ------------------------------- */
+QK_EXPORT double  qk_mcm_quasi_monte_carlo_sobol_price(double spot, double strike, double t, double vol,
+                                                       double r, double q, int32_t option_type,
+                                                       int32_t paths);
+QK_EXPORT double  qk_mcm_quasi_monte_carlo_halton_price(double spot, double strike, double t, double vol,
+                                                        double r, double q, int32_t option_type,
+                                                        int32_t paths);
+QK_EXPORT double  qk_mcm_multilevel_monte_carlo_price(double spot, double strike, double t, double vol,
+                                                      double r, double q, int32_t option_type,
+                                                      int32_t base_paths, int32_t levels, int32_t base_steps,
+                                                      uint64_t seed);
+QK_EXPORT double  qk_mcm_control_variates_price(double spot, double strike, double t, double vol,
+                                                double r, double q, int32_t option_type,
+                                                int32_t paths, uint64_t seed);
+QK_EXPORT double  qk_mcm_stratified_sampling_price(double spot, double strike, double t, double vol,
+                                                   double r, double q, int32_t option_type,
+                                                   int32_t paths, uint64_t seed);
+QK_EXPORT int32_t qk_mcm_quasi_monte_carlo_sobol_price_batch(const double* spot, const double* strike,
+                                                             const double* t, const double* vol,
+                                                             const double* r, const double* q,
+                                                             const int32_t* option_type, const int32_t* paths,
+                                                             int32_t n, double* out_prices);
+QK_EXPORT int32_t qk_mcm_quasi_monte_carlo_halton_price_batch(const double* spot, const double* strike,
+                                                              const double* t, const double* vol,
+                                                              const double* r, const double* q,
+                                                              const int32_t* option_type, const int32_t* paths,
+                                                              int32_t n, double* out_prices);
+QK_EXPORT int32_t qk_mcm_multilevel_monte_carlo_price_batch(const double* spot, const double* strike,
+                                                            const double* t, const double* vol,
+                                                            const double* r, const double* q,
+                                                            const int32_t* option_type, const int32_t* base_paths,
+                                                            const int32_t* levels, const int32_t* base_steps,
+                                                            const uint64_t* seed,
+                                                            int32_t n, double* out_prices);
+QK_EXPORT int32_t qk_mcm_control_variates_price_batch(const double* spot, const double* strike,
+                                                      const double* t, const double* vol,
+                                                      const double* r, const double* q,
+                                                      const int32_t* option_type, const int32_t* paths,
+                                                      const uint64_t* seed,
+                                                      int32_t n, double* out_prices);
+QK_EXPORT int32_t qk_mcm_stratified_sampling_price_batch(const double* spot, const double* strike,
+                                                         const double* t, const double* vol,
+                                                         const double* r, const double* q,
+                                                         const int32_t* option_type, const int32_t* paths,
+                                                         const uint64_t* seed,
+                                                         int32_t n, double* out_prices);
+
 /* ---- importance sampling (reference qk_api.h:288-290, 335-340) ---------- */
 QK_EXPORT double  qk_mcm_importance_sampling_price(double spot, double strike, double t, double vol,
                                                    double r, double q, int32_t option_type,
@@ -238,6 +286,33 @@ QK_EXPORT int32_t qk_agm_likelihood_ratio_delta_batch_se(const double* spot, con
         const double* t, const double* vol, const double* r, const double* q,
         const int32_t* option_type, const int32_t* paths, const uint64_t* seed,
         const double* weight_clip, int32_t n, double* out_prices, double* out_stderr);
+
+/* Standard errors of control variates (residual of the control-variate
+ * fit, per path), stratified sampling (per path, the iid formula, which
+ * overstates a stratified estimator's error), multilevel MC (the levels'
+ * errors in quadrature) and quasi-MC (NaN: a point set has no sampling
+ * error). */
+QK_EXPORT int32_t qk_mcm_control_variates_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const uint64_t* seed,
+        int32_t n, double* out_prices, double* out_stderr);
+QK_EXPORT int32_t qk_mcm_stratified_sampling_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, const uint64_t* seed,
+        int32_t n, double* out_prices, double* out_stderr);
+QK_EXPORT int32_t qk_mcm_quasi_monte_carlo_sobol_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, int32_t n, double* out_prices,
+        double* out_stderr);
+QK_EXPORT int32_t qk_mcm_quasi_monte_carlo_halton_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* paths, int32_t n, double* out_prices,
+        double* out_stderr);
+QK_EXPORT int32_t qk_mcm_multilevel_monte_carlo_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* base_paths, const int32_t* levels,
+        const int32_t* base_steps, const uint64_t* seed, int32_t n, double* out_prices,
+        double* out_stderr);
 
 /* Mixed-model uneven batch: one persistent launch over options of different
  * models (QK_MODEL_*), path counts and step counts.  vol is used by the GBM

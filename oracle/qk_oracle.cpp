@@ -1063,3 +1063,296 @@ ORC_EXPORT double orc_px_heston_lr_delta(double spot, double strike, double t, d
     *se = se_of(mo, double(pairs), disc) * pairs / double(paths);
     return disc * sum / double(paths);
 }
+
+// ===========================================================================
+// 5. Remaining variance-reduction estimators of the MC family: control
+//    variates, stratified sampling, Sobol / Halton QMC and multilevel MC
+//    (monte_carlo_methods/{control_variates,stratified_sampling,
+//    quasi_monte_carlo,multilevel_monte_carlo}), on the reference's stream and
+//    on the GPU's Philox stream.
+// ===========================================================================
+namespace {
+
+// Acklam's rational approximation of the inverse normal CDF (P. J. Acklam,
+// 2003; relative error < 1.15e-9), as monte_carlo_methods/common/
+// internal_util.h:43-83 evaluates it (same coefficients, branches and
+// Horner order).
+double acklam(double p) {
+    if (p <= 0.0 || p >= 1.0) return qnan();
+    const double a[6] = {-3.969683028665376e+01, 2.209460984245205e+02, -2.759285104469687e+02,
+                         1.383577518672690e+02, -3.066479806614716e+01, 2.506628277459239e+00};
+    const double b[5] = {-5.447609879822406e+01, 1.615858368580409e+02, -1.556989798598866e+02,
+                         6.680131188771972e+01, -1.328068155288572e+01};
+    const double c[6] = {-7.784894002430293e-03, -3.223964580411365e-01, -2.400758277161838e+00,
+                         -2.549732539343734e+00, 4.374664141464968e+00, 2.938163982698783e+00};
+    const double d[4] = {7.784695709041462e-03, 3.224671290700398e-01, 2.445134137142996e+00,
+                         3.754408661907416e+00};
+    const double lo = 0.02425, hi = 1.0 - lo;
+    auto tail = [&](double qq) {
+        return (((((c[0] * qq + c[1]) * qq + c[2]) * qq + c[3]) * qq + c[4]) * qq + c[5]) /
+               ((((d[0] * qq + d[1]) * qq + d[2]) * qq + d[3]) * qq + 1.0);
+    };
+    if (p < lo) return tail(std::sqrt(-2.0 * std::log(p)));
+    if (p > hi) return -tail(std::sqrt(-2.0 * std::log(1.0 - p)));
+    const double qq = p - 0.5, rr = qq * qq;
+    return (((((a[0] * rr + a[1]) * rr + a[2]) * rr + a[3]) * rr + a[4]) * rr + a[5]) * qq /
+           (((((b[0] * rr + b[1]) * rr + b[2]) * rr + b[3]) * rr + b[4]) * rr + 1.0);
+}
+
+uint32_t bit_reverse(uint32_t x) {
+    uint32_t y = 0;
+    for (int i = 0; i < 32; ++i) y |= ((x >> i) & 1u) << (31 - i);
+    return y;
+}
+
+// internal_util.h:96-110: direction number of bit j is 1 << (31 - j), so the
+// Gray-code XOR sum is the bit reverse of gray(i).
+double sobol_point(uint32_t i) {
+    if (i == 0) return 0.0;
+    return (double(bit_reverse(i ^ (i >> 1))) + 0.5) / 4294967296.0;
+}
+
+// internal_util.h:112-122 with base 2 (van der Corput): exact binary fractions.
+double halton2_point(uint32_t i) { return double(bit_reverse(i)) / 4294967296.0; }
+
+double clamp_u(double u) { return std::min(std::max(u, 1e-12), 1.0 - 1e-12); }
+
+// mcm internal_util.h:32-36: (spot, t, r, q, vol, z) argument order.
+double gbm_terminal_mcm(double spot, double t, double r, double q, double vol, double z) {
+    const double drift = (r - q - 0.5 * vol * vol) * t;
+    const double diffusion = vol * std::sqrt(std::max(t, 0.0)) * z;
+    return spot * std::exp(drift + diffusion);
+}
+
+// Uniform k of the Philox uniform stream: word k%4 of block (k/4, 0, 0, 2),
+// (w + 0.5) 2^-32.
+double philox_uniform(uint64_t seed, uint32_t k) {
+    const uint32_t in[4] = {k >> 2, 0u, 0u, 2u};
+    const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    uint32_t w[4];
+    philox10(in, key, w);
+    return (double(w[k & 3u]) + 0.5) * 0x1p-32;
+}
+
+// MLMC draw d of path p at level l: component d%4 of block (p, d/4, 0, l).
+double philox_level_normal(uint64_t seed, uint32_t p, uint32_t d, uint32_t level) {
+    double z[4];
+    philox_normals4(p, d >> 2, level, seed, z);
+    return z[d & 3u];
+}
+
+// The control-variate combination of control_variates.cpp:42-53 from
+// population moments; returns the estimate and the residual standard error.
+double cv_combine(double sum_y, double sum_x, double sum_xx, double sum_xy, double sum_yy,
+                  int32_t paths, double expected_x, double* se) {
+    const double n = double(paths);
+    const double mean_y = sum_y / n;
+    const double mean_x = sum_x / n;
+    const double var_x = (sum_xx / n) - mean_x * mean_x;
+    const double var_y = (sum_yy / n) - mean_y * mean_y;
+    if (var_x <= kEps) {
+        if (se) *se = std::sqrt(std::max(var_y, 0.0) / (n - 1.0));
+        return mean_y;
+    }
+    const double cov_xy = (sum_xy / n) - mean_x * mean_y;
+    const double beta = cov_xy / var_x;
+    if (se) *se = std::sqrt(std::max(var_y - cov_xy * beta, 0.0) / (n - 1.0));
+    return mean_y - beta * (mean_x - expected_x);
+}
+
+}  // namespace
+
+// control_variates.cpp:11-54 (one draw per path; control x = e^{-rT} S_T with
+// known mean S e^{-qT}).
+ORC_EXPORT double orc_control_variates(double spot, double strike, double t, double vol, double r,
+                                       double q, int32_t ot, int32_t paths, uint64_t seed) {
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths <= 2) return qnan();
+    if (t <= kEps) return intrinsic(spot, strike, ot);
+    MtStream g(seed);
+    const double disc = std::exp(-r * t);
+    const double expected_x = spot * std::exp(-q * t);
+    double sy = 0.0, sx = 0.0, sxx = 0.0, sxy = 0.0;
+    for (int32_t i = 0; i < paths; ++i) {
+        const double st = gbm_terminal_mcm(spot, t, r, q, vol, g());
+        const double x = disc * st;
+        const double y = disc * intrinsic(st, strike, ot);
+        sy += y;
+        sx += x;
+        sxx += x * x;
+        sxy += x * y;
+    }
+    return cv_combine(sy, sx, sxx, sxy, 0.0, paths, expected_x, nullptr);
+}
+
+// stratified_sampling.cpp:11-35: u_i = (i + U_i)/n, U_i from
+// std::uniform_real_distribution<double>(0, 1) on the option's mt19937_64.
+ORC_EXPORT double orc_stratified(double spot, double strike, double t, double vol, double r,
+                                 double q, int32_t ot, int32_t paths, uint64_t seed) {
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths <= 1) return qnan();
+    if (t <= kEps) return intrinsic(spot, strike, ot);
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> unif(0.0, 1.0);
+    const double disc = std::exp(-r * t);
+    double sum = 0.0;
+    for (int32_t i = 0; i < paths; ++i) {
+        const double u = clamp_u((double(i) + unif(rng)) / double(paths));
+        sum += intrinsic(gbm_terminal_mcm(spot, t, r, q, vol, acklam(u)), strike, ot);
+    }
+    return disc * (sum / double(paths));
+}
+
+// The uniforms of the reference's stratified stream (for the product's
+// validation-stream check).
+ORC_EXPORT void orc_mt_uniforms(uint64_t seed, int64_t count, double* out) {
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> unif(0.0, 1.0);
+    for (int64_t i = 0; i < count; ++i) out[i] = unif(rng);
+}
+
+ORC_EXPORT double orc_acklam(double p) { return acklam(p); }
+ORC_EXPORT double orc_sobol_point(uint32_t i) { return sobol_point(i); }
+ORC_EXPORT double orc_halton2_point(uint32_t i) { return halton2_point(i); }
+
+// quasi_monte_carlo.cpp:10-56: points i = 1..paths, no seed.  kind 0 Sobol,
+// 1 Halton (base 2).
+ORC_EXPORT double orc_qmc(int32_t kind, double spot, double strike, double t, double vol, double r,
+                          double q, int32_t ot, int32_t paths) {
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths <= 1) return qnan();
+    if (t <= kEps) return intrinsic(spot, strike, ot);
+    const double disc = std::exp(-r * t);
+    double sum = 0.0;
+    for (int32_t i = 1; i <= paths; ++i) {
+        const double u0 = kind == 0 ? sobol_point(uint32_t(i)) : halton2_point(uint32_t(i));
+        const double z = acklam(clamp_u(u0));
+        sum += intrinsic(gbm_terminal_mcm(spot, t, r, q, vol, z), strike, ot);
+    }
+    return disc * (sum / double(paths));
+}
+
+// multilevel_monte_carlo.cpp:10-72.  Level l: max(16, base_paths >> l) paths
+// of base_steps << l Euler steps; levels >= 1 add fine - coarse with the
+// coarse path driven by (z1 + z2)/sqrt 2 over two fine steps.  `draw(l, p, d)`
+// returns draw d of path p at level l.
+template <class Draw>
+static double mlmc_impl(double spot, double strike, double t, double vol, double r, double q,
+                        int32_t ot, int32_t base_paths, int32_t levels, int32_t base_steps,
+                        Draw&& draw, double* se) {
+    double estimate = 0.0, var_sum = 0.0;
+    for (int32_t level = 0; level < levels; ++level) {
+        const int32_t n_level = std::max(16, base_paths >> level);
+        const int32_t fine_steps = base_steps << level;
+        const double dt_fine = t / double(fine_steps);
+        const double sqrt_dt_fine = std::sqrt(dt_fine);
+        const double disc = std::exp(-r * t);
+        double level_sum = 0.0;
+        Moments mo;
+        for (int32_t i = 0; i < n_level; ++i) {
+            double unit;
+            if (level == 0) {
+                double s = spot;
+                for (int32_t j = 0; j < fine_steps; ++j) {
+                    const double z = draw(level, i, j);
+                    s += (r - q) * s * dt_fine + vol * s * sqrt_dt_fine * z;
+                    s = std::max(1e-12, s);
+                }
+                unit = intrinsic(s, strike, ot);
+            } else {
+                double s_fine = spot, s_coarse = spot;
+                for (int32_t cs = 0; cs < fine_steps / 2; ++cs) {
+                    const double z1 = draw(level, i, 2 * cs);
+                    const double z2 = draw(level, i, 2 * cs + 1);
+                    s_fine += (r - q) * s_fine * dt_fine + vol * s_fine * sqrt_dt_fine * z1;
+                    s_fine = std::max(1e-12, s_fine);
+                    s_fine += (r - q) * s_fine * dt_fine + vol * s_fine * sqrt_dt_fine * z2;
+                    s_fine = std::max(1e-12, s_fine);
+                    const double zc = (z1 + z2) * M_SQRT1_2;
+                    const double dt_coarse = 2.0 * dt_fine;
+                    s_coarse += (r - q) * s_coarse * dt_coarse +
+                                vol * s_coarse * std::sqrt(dt_coarse) * zc;
+                    s_coarse = std::max(1e-12, s_coarse);
+                }
+                unit = intrinsic(s_fine, strike, ot) - intrinsic(s_coarse, strike, ot);
+            }
+            level_sum += unit;
+            mo.add(unit);
+        }
+        estimate += disc * (level_sum / double(n_level));
+        const double sl = se_of(mo, double(n_level), disc);
+        var_sum += sl * sl;
+    }
+    if (se) *se = std::sqrt(var_sum);
+    return estimate;
+}
+
+static bool mlmc_ok(double spot, double strike, double t, double vol, double r, double q,
+                    int32_t ot, int32_t base_paths, int32_t levels, int32_t base_steps) {
+    return common_ok(spot, strike, t, vol, r, q, ot) && base_paths > 1 && levels > 0 &&
+           base_steps > 0;
+}
+
+ORC_EXPORT double orc_mlmc(double spot, double strike, double t, double vol, double r, double q,
+                           int32_t ot, int32_t base_paths, int32_t levels, int32_t base_steps,
+                           uint64_t seed) {
+    if (!mlmc_ok(spot, strike, t, vol, r, q, ot, base_paths, levels, base_steps)) return qnan();
+    if (t <= kEps) return intrinsic(spot, strike, ot);
+    MtStream g(seed);  // one stream, consumed level by level, path by path
+    return mlmc_impl(spot, strike, t, vol, r, q, ot, base_paths, levels, base_steps,
+                     [&](int32_t, int32_t, int32_t) { return g(); }, nullptr);
+}
+
+// ---- Philox-stream versions with standard errors ----
+ORC_EXPORT double orc_px_control_variates(double spot, double strike, double t, double vol,
+                                          double r, double q, int32_t ot, int32_t paths,
+                                          uint64_t seed, double* se) {
+    *se = qnan();
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths <= 2) return qnan();
+    *se = 0.0;
+    if (t <= kEps) return intrinsic(spot, strike, ot);
+    const double disc = std::exp(-r * t);
+    const double expected_x = spot * std::exp(-q * t);
+    double sy = 0.0, sx = 0.0, sxx = 0.0, sxy = 0.0, syy = 0.0;
+    for (int32_t i = 0; i < paths; ++i) {
+        const double st =
+            gbm_terminal_mcm(spot, t, r, q, vol, philox_terminal_normal(seed, uint32_t(i)));
+        const double x = disc * st;
+        const double y = disc * intrinsic(st, strike, ot);
+        sy += y;
+        sx += x;
+        sxx += x * x;
+        sxy += x * y;
+        syy += y * y;
+    }
+    return cv_combine(sy, sx, sxx, sxy, syy, paths, expected_x, se);
+}
+
+ORC_EXPORT double orc_px_stratified(double spot, double strike, double t, double vol, double r,
+                                    double q, int32_t ot, int32_t paths, uint64_t seed,
+                                    double* se) {
+    *se = qnan();
+    if (!common_ok(spot, strike, t, vol, r, q, ot) || paths <= 1) return qnan();
+    *se = 0.0;
+    if (t <= kEps) return intrinsic(spot, strike, ot);
+    const double disc = std::exp(-r * t);
+    Moments mo;
+    for (int32_t i = 0; i < paths; ++i) {
+        const double u =
+            clamp_u((double(i) + philox_uniform(seed, uint32_t(i))) / double(paths));
+        mo.add(intrinsic(gbm_terminal_mcm(spot, t, r, q, vol, acklam(u)), strike, ot));
+    }
+    *se = se_of(mo, double(paths), disc);
+    return disc * (mo.s1 / double(paths));
+}
+
+ORC_EXPORT double orc_px_mlmc(double spot, double strike, double t, double vol, double r,
+                              double q, int32_t ot, int32_t base_paths, int32_t levels,
+                              int32_t base_steps, uint64_t seed, double* se) {
+    *se = qnan();
+    if (!mlmc_ok(spot, strike, t, vol, r, q, ot, base_paths, levels, base_steps)) return qnan();
+    *se = 0.0;
+    if (t <= kEps) return intrinsic(spot, strike, ot);
+    return mlmc_impl(spot, strike, t, vol, r, q, ot, base_paths, levels, base_steps,
+                     [&](int32_t l, int32_t p, int32_t d) {
+                         return philox_level_normal(seed, uint32_t(p), uint32_t(d), uint32_t(l));
+                     },
+                     se);
+}

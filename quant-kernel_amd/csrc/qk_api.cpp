@@ -12,6 +12,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
+#include <algorithm>
 
 #include "qk_engine.hpp"
 #include "qk_lsmc.hpp"
@@ -152,6 +154,99 @@ int32_t heston_lr_batch(const double* spot, const double* strike, const double* 
     hb.seed = seed;
     hb.aux = wclip;
     return run(hb, n, out, se);
+}
+
+// Quasi-MC has no seed in the reference's signature; the engine takes one
+// per element and ignores it for point sets.
+int32_t qmc_batch(int32_t method, const double* spot, const double* strike, const double* t,
+                  const double* vol, const double* r, const double* q, const int32_t* ot,
+                  const int32_t* paths, int32_t n, double* out, double* se) {
+    std::vector<uint64_t> seed(static_cast<size_t>(n), 0);
+    return gbm_batch(method, spot, strike, t, vol, r, q, ot, paths, nullptr, seed.data(), n, out,
+                     se);
+}
+
+// Multilevel MC (multilevel_monte_carlo.cpp:10-72): each option becomes one
+// element per level (paths max(16, base_paths >> l), steps base_steps << l,
+// aux = l) in one batch; the levels' discounted means are then summed in
+// level order, and their standard errors in quadrature.  base_paths <= 1,
+// levels <= 0 or base_steps <= 0 give NaN (multilevel_monte_carlo.cpp:14-17);
+// so do level counts whose finest step count overflows int32 (the reference
+// shifts past 2^31 there).
+int32_t mlmc_batch(const double* spot, const double* strike, const double* t, const double* vol,
+                   const double* r, const double* q, const int32_t* ot, const int32_t* base_paths,
+                   const int32_t* levels, const int32_t* base_steps, const uint64_t* seed,
+                   int32_t n, double* out, double* se) {
+    std::vector<int64_t> first(static_cast<size_t>(n));
+    std::vector<int32_t> count(static_cast<size_t>(n));
+    int64_t total = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t L = levels[i];
+        const bool ok = base_paths[i] > 1 && L > 0 && base_steps[i] > 0 && L <= 31 &&
+                        (int64_t(base_steps[i]) << (L - 1)) <= 0x7fffffffLL;
+        count[i] = ok ? L : 0;
+        first[i] = total;
+        total += count[i];
+    }
+    if (total > 0x7fffffffLL) {
+        set_msg("multilevel batch too large: more than 2^31-1 levels in total");
+        return QK_ERR_INVALID_INPUT;
+    }
+    const size_t m = static_cast<size_t>(total);
+    std::vector<double> es(m), ek(m), et(m), ev(m), er(m), eq(m), ea(m), sp(m), ss(m);
+    std::vector<int32_t> eot(m), ep(m), est(m);
+    std::vector<uint64_t> eseed(m);
+    std::vector<uint8_t> cont(m);
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t l = 0; l < count[i]; ++l) {
+            const size_t j = static_cast<size_t>(first[i] + l);
+            es[j] = spot[i];
+            ek[j] = strike[i];
+            et[j] = t[i];
+            ev[j] = vol[i];
+            er[j] = r[i];
+            eq[j] = q[i];
+            eot[j] = ot[i];
+            ep[j] = std::max(16, base_paths[i] >> l);
+            est[j] = base_steps[i] << l;
+            eseed[j] = seed[i];
+            ea[j] = double(l);
+            cont[j] = l > 0;
+        }
+    if (total > 0) {
+        qk::HostBatch hb;
+        hb.uniform_method = qk::M_MLMC_LEVEL;
+        hb.spot = es.data();
+        hb.strike = ek.data();
+        hb.t = et.data();
+        hb.vol = ev.data();
+        hb.r = er.data();
+        hb.q = eq.data();
+        hb.ot = eot.data();
+        hb.paths = ep.data();
+        hb.steps = est.data();
+        hb.seed = eseed.data();
+        hb.aux = ea.data();
+        hb.stream_continues = cont.data();
+        const int32_t rc = run(hb, int32_t(total), sp.data(), ss.data());
+        if (rc != QK_OK) return rc;
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        if (count[i] == 0) {
+            out[i] = qk::qnan_d();
+            if (se) se[i] = qk::qnan_d();
+            continue;
+        }
+        double est = 0.0, var = 0.0;
+        for (int32_t l = 0; l < count[i]; ++l) {
+            const size_t j = static_cast<size_t>(first[i] + l);
+            est += sp[j];
+            var += ss[j] * ss[j];
+        }
+        out[i] = est;
+        if (se) se[i] = std::sqrt(var);
+    }
+    return QK_OK;
 }
 
 double scalar_of(int32_t rc, double v) { return rc == QK_OK ? v : qk::qnan_d(); }
@@ -496,6 +591,172 @@ double qk_agm_likelihood_ratio_delta(double spot, double strike, double t, doubl
     double out = 0.0;
     const int32_t rc = gbm_aux_batch(qk::M_LR, &spot, &strike, &t, &vol, &r, &q, &option_type,
                                      &paths, &weight_clip, &seed, 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- control variates (reference qk_api.cpp:926-934, 1073-1089) ----
+int32_t qk_mcm_control_variates_price_batch_se(const double* spot, const double* strike, const double* t,
+        const double* vol, const double* r, const double* q, const int32_t* option_type,
+        const int32_t* paths, const uint64_t* seed, int32_t n, double* out_prices,
+        double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_batch(qk::M_CONTROL, spot, strike, t, vol, r, q, option_type, paths, nullptr, seed, n,
+                     out_prices, out_stderr);
+}
+
+int32_t qk_mcm_control_variates_price_batch(const double* spot, const double* strike, const double* t,
+        const double* vol, const double* r, const double* q, const int32_t* option_type,
+        const int32_t* paths, const uint64_t* seed, int32_t n, double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_batch(qk::M_CONTROL, spot, strike, t, vol, r, q, option_type, paths, nullptr, seed, n,
+                     out_prices, nullptr);
+}
+
+double qk_mcm_control_variates_price(double spot, double strike, double t, double vol, double r, double q,
+        int32_t option_type, int32_t paths, uint64_t seed) {
+    double out = 0.0;
+    const int32_t rc = gbm_batch(qk::M_CONTROL, &spot, &strike, &t, &vol, &r, &q, &option_type, &paths,
+                                 nullptr, &seed, 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- stratified sampling (reference qk_api.cpp:942-950, 1109-1125) ----
+int32_t qk_mcm_stratified_sampling_price_batch_se(const double* spot, const double* strike, const double* t,
+        const double* vol, const double* r, const double* q, const int32_t* option_type,
+        const int32_t* paths, const uint64_t* seed, int32_t n, double* out_prices,
+        double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_batch(qk::M_STRATIFIED, spot, strike, t, vol, r, q, option_type, paths, nullptr, seed, n,
+                     out_prices, out_stderr);
+}
+
+int32_t qk_mcm_stratified_sampling_price_batch(const double* spot, const double* strike, const double* t,
+        const double* vol, const double* r, const double* q, const int32_t* option_type,
+        const int32_t* paths, const uint64_t* seed, int32_t n, double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(seed), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return gbm_batch(qk::M_STRATIFIED, spot, strike, t, vol, r, q, option_type, paths, nullptr, seed, n,
+                     out_prices, nullptr);
+}
+
+double qk_mcm_stratified_sampling_price(double spot, double strike, double t, double vol, double r, double q,
+        int32_t option_type, int32_t paths, uint64_t seed) {
+    double out = 0.0;
+    const int32_t rc = gbm_batch(qk::M_STRATIFIED, &spot, &strike, &t, &vol, &r, &q, &option_type, &paths,
+                                 nullptr, &seed, 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- quasi-MC, Sobol (reference qk_api.cpp:893-899, 1004-1019) ----
+int32_t qk_mcm_quasi_monte_carlo_sobol_price_batch_se(const double* spot, const double* strike, const double* t,
+        const double* vol, const double* r, const double* q, const int32_t* option_type,
+        const int32_t* paths, int32_t n, double* out_prices, double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(out_prices), QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return qmc_batch(qk::M_SOBOL, spot, strike, t, vol, r, q, option_type, paths, n, out_prices,
+                     out_stderr);
+}
+
+int32_t qk_mcm_quasi_monte_carlo_sobol_price_batch(const double* spot, const double* strike, const double* t,
+        const double* vol, const double* r, const double* q, const int32_t* option_type,
+        const int32_t* paths, int32_t n, double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return qmc_batch(qk::M_SOBOL, spot, strike, t, vol, r, q, option_type, paths, n, out_prices,
+                     nullptr);
+}
+
+double qk_mcm_quasi_monte_carlo_sobol_price(double spot, double strike, double t, double vol, double r, double q,
+        int32_t option_type, int32_t paths) {
+    double out = 0.0;
+    const int32_t rc = qmc_batch(qk::M_SOBOL, &spot, &strike, &t, &vol, &r, &q, &option_type, &paths,
+                                 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- quasi-MC, Halton (reference qk_api.cpp:901-907, 1020-1035) ----
+int32_t qk_mcm_quasi_monte_carlo_halton_price_batch_se(const double* spot, const double* strike, const double* t,
+        const double* vol, const double* r, const double* q, const int32_t* option_type,
+        const int32_t* paths, int32_t n, double* out_prices, double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(out_prices), QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return qmc_batch(qk::M_HALTON, spot, strike, t, vol, r, q, option_type, paths, n, out_prices,
+                     out_stderr);
+}
+
+int32_t qk_mcm_quasi_monte_carlo_halton_price_batch(const double* spot, const double* strike, const double* t,
+        const double* vol, const double* r, const double* q, const int32_t* option_type,
+        const int32_t* paths, int32_t n, double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(paths), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return qmc_batch(qk::M_HALTON, spot, strike, t, vol, r, q, option_type, paths, n, out_prices,
+                     nullptr);
+}
+
+double qk_mcm_quasi_monte_carlo_halton_price(double spot, double strike, double t, double vol, double r, double q,
+        int32_t option_type, int32_t paths) {
+    double out = 0.0;
+    const int32_t rc = qmc_batch(qk::M_HALTON, &spot, &strike, &t, &vol, &r, &q, &option_type, &paths,
+                                 1, &out, nullptr);
+    return scalar_of(rc, out);
+}
+
+// ---- multilevel MC (reference qk_api.cpp:909-916, 1036-1053) ----
+int32_t qk_mcm_multilevel_monte_carlo_price_batch_se(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* base_paths, const int32_t* levels,
+        const int32_t* base_steps, const uint64_t* seed, int32_t n, double* out_prices,
+        double* out_stderr) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(base_paths), QK_P(levels), QK_P(base_steps),
+                     QK_P(seed), QK_P(out_prices), QK_P(out_stderr)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return mlmc_batch(spot, strike, t, vol, r, q, option_type, base_paths, levels, base_steps,
+                      seed, n, out_prices, out_stderr);
+}
+
+int32_t qk_mcm_multilevel_monte_carlo_price_batch(const double* spot, const double* strike,
+        const double* t, const double* vol, const double* r, const double* q,
+        const int32_t* option_type, const int32_t* base_paths, const int32_t* levels,
+        const int32_t* base_steps, const uint64_t* seed, int32_t n, double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(option_type), QK_P(base_paths), QK_P(levels), QK_P(base_steps),
+                     QK_P(seed), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    return mlmc_batch(spot, strike, t, vol, r, q, option_type, base_paths, levels, base_steps,
+                      seed, n, out_prices, nullptr);
+}
+
+double qk_mcm_multilevel_monte_carlo_price(double spot, double strike, double t, double vol,
+                                           double r, double q, int32_t option_type,
+                                           int32_t base_paths, int32_t levels, int32_t base_steps,
+                                           uint64_t seed) {
+    double out = 0.0;
+    const int32_t rc = mlmc_batch(&spot, &strike, &t, &vol, &r, &q, &option_type, &base_paths,
+                                  &levels, &base_steps, &seed, 1, &out, nullptr);
     return scalar_of(rc, out);
 }
 

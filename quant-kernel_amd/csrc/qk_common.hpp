@@ -35,14 +35,21 @@ enum Method : int32_t {
     M_PATHWISE = 7,    // pathwise_derivative.cpp:11-40 (estimate_terminal)
     M_LR = 8,          // likelihood_ratio.cpp:12-42 (estimate_terminal_antithetic)
     M_HESTON_LR = 9,   // heston_lr_delta.cpp:41-132 (antithetic Heston pairs)
+    M_CONTROL = 10,    // control_variates.cpp:11-54 (control e^{-rT} S_T)
+    M_STRATIFIED = 11, // stratified_sampling.cpp:11-35 (one uniform per stratum)
+    M_SOBOL = 12,      // quasi_monte_carlo.cpp:10-32 (1-D Sobol points)
+    M_HALTON = 13,     // quasi_monte_carlo.cpp:34-56 (base-2 Halton points)
+    M_MLMC_LEVEL = 14, // one level of multilevel_monte_carlo.cpp:10-72 (aux = level)
 };
 
 // Terminal methods simulate S_T in one draw; the antithetic ones use a pair
 // (z, -z) per unit.
 QK_HD bool method_is_terminal(int32_t m) {
     return m == M_STANDARD || m == M_ANTITHETIC || m == M_IMPORTANCE || m == M_PATHWISE ||
-           m == M_LR;
+           m == M_LR || m == M_CONTROL || m == M_STRATIFIED || m == M_SOBOL || m == M_HALTON;
 }
+// Deterministic point sets: no stream, and no sampling-error estimate.
+QK_HD bool method_is_qmc(int32_t m) { return m == M_SOBOL || m == M_HALTON; }
 QK_HD bool method_is_antithetic_terminal(int32_t m) {
     return m == M_STANDARD || m == M_ANTITHETIC || m == M_LR;
 }
@@ -77,7 +84,7 @@ struct RawOption {
     double v0, kappa, theta, sigma, rho;
     int32_t ot, paths, steps;
     uint64_t seed;
-    double aux;  // importance sampling: shift; LR deltas: weight clip
+    double aux;  // importance sampling: shift; LR deltas: weight clip; MLMC: level
 };
 
 // Prepared option: everything a path kernel and the finaliser need.
@@ -105,6 +112,8 @@ struct alignas(16) OptionRec {
     //  aux[1]: e^{-qT}, the delta clamp (pathwise and LR delta)
     //  aux[2]: LR score denominator: vol sqrt(T) spot (GBM) or
     //          sqrt(v0) sqrt(dt) spot (Heston; 0 when sqrt(v0) <= 1e-15: score 0)
+    //  control variates: aux[0] = E[S_T] = S e^{(r-q)T} (the control's mean,
+    //          undiscounted); MLMC level: aux[0] = level
     double aux[4];
 };
 static_assert(sizeof(OptionRec) == 192, "OptionRec layout");
@@ -219,6 +228,46 @@ QK_HD int32_t prepare_option(const RawOption& in, int32_t tile_units, OptionRec&
         } else {
             rec.units = in.paths;
         }
+    } else if (in.method == M_CONTROL || in.method == M_STRATIFIED || in.method == M_SOBOL ||
+               in.method == M_HALTON) {
+        // control_variates.cpp:14-19 (paths > 2), stratified_sampling.cpp:14-19,
+        // quasi_monte_carlo.cpp:13-18 (paths > 1); no vol <= eps branch
+        const int32_t min_paths = in.method == M_CONTROL ? 3 : 2;
+        if (!common_ok(s, k, t, in.vol, r, q, ot) || in.paths < min_paths) return FAM_NONE;
+        if (t <= kEps) {
+            rec.closed = intrinsic(s, k, ot);
+            return FAM_NONE;
+        }
+        rec.disc = exp(-r * t);
+        rec.c[0] = (r - q - 0.5 * in.vol * in.vol) * t;
+        rec.c[1] = in.vol * sqrt(t);
+        const double fwd = s * exp((r - q) * t);
+        rec.shift = intrinsic(fwd, k, ot);
+        rec.aux[0] = fwd;
+        rec.family = FAM_TERMINAL;
+        rec.units = in.paths;
+        rec.steps = 1;
+    } else if (in.method == M_MLMC_LEVEL) {
+        // one level of multilevel_monte_carlo.cpp:24-66: paths = max(16,
+        // base_paths >> level) and steps = base_steps << level are set by the
+        // host, which also validates base_paths, levels and base_steps
+        if (!common_ok(s, k, t, in.vol, r, q, ot) || in.paths <= 1 || in.steps <= 0)
+            return FAM_NONE;
+        const bool fine_only = !(in.aux > 0.0);  // level 0
+        if (t <= kEps) {
+            rec.closed = fine_only ? intrinsic(s, k, ot) : 0.0;  // the levels sum to intrinsic
+            return FAM_NONE;
+        }
+        const double dt = t / double(in.steps);
+        rec.disc = exp(-r * t);
+        rec.c[0] = 1.0 + (r - q) * dt;
+        rec.c[1] = in.vol * sqrt(dt);
+        rec.c[2] = 1.0 + (r - q) * (2.0 * dt);
+        rec.c[3] = in.vol * sqrt(2.0 * dt);
+        rec.aux[0] = in.aux;
+        rec.shift = fine_only ? intrinsic(s * exp((r - q) * t), k, ot) : 0.0;
+        rec.family = FAM_STEPWISE;
+        rec.units = in.paths;
     } else if (in.method == M_IMPORTANCE) {
         // importance_sampling.cpp:15-20
         if (!common_ok(s, k, t, in.vol, r, q, ot) || in.paths <= 1 || !finite_d(in.aux))
@@ -287,10 +336,12 @@ QK_HD int32_t prepare_option(const RawOption& in, int32_t tile_units, OptionRec&
 // Validation-mode stream geometry: units that draw from the host stream
 // (paths, pairs, plus an odd path) and draws per unit.
 QK_HD int64_t stream_units(const OptionRec& rec) {
-    if (rec.family == FAM_TERMINAL)
+    if (rec.family == FAM_TERMINAL) {
+        if (method_is_qmc(rec.method)) return 0;
         return method_is_antithetic_terminal(rec.method)
                    ? int64_t(rec.paths / 2) + (rec.paths & 1)
                    : int64_t(rec.paths);
+    }
     if (rec.family == FAM_STEPWISE)
         return rec.method == M_HESTON_LR ? int64_t(rec.paths / 2) + (rec.paths & 1)
                                          : int64_t(rec.paths);
@@ -306,12 +357,16 @@ QK_HD int64_t stream_normals(const OptionRec& rec) {
     return stream_units(rec) * stream_draws_per_unit(rec);
 }
 
-// Per-tile partial sums of centred payoff units.
-struct alignas(32) Partial {
+// Per-tile partial sums of centred payoff units (64 bytes: one HBM line).
+struct alignas(64) Partial {
     double s1;     // sum of (unit - shift)
     double s2;     // sum of (unit - shift)^2
-    double extra;  // terminal: payoff of the odd extra path (0 otherwise)
-    double pad;
+    double extra;  // payoff of an odd extra path (0 otherwise)
+    double x1;     // control variates: sum of (S_T - E[S_T])
+    double x2;     //   sum of (S_T - E[S_T])^2
+    double xy;     //   sum of (S_T - E[S_T]) (unit - shift)
+    double pad[2];
 };
+static_assert(sizeof(Partial) == 64, "Partial layout");
 
 }  // namespace qk

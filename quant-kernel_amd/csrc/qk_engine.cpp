@@ -215,31 +215,46 @@ bool device_info(std::string* name, int32_t* cus, std::string* err) {
     return true;
 }
 
-// Builds the validation-mode stream: per option the host mt19937_64 normal
-// sequence in the reference's draw order (paths outer, draws inner;
-// mc_engine.h:84-136), stored draw-major for the kernels.
+// Builds the validation-mode stream: per option the host mt19937_64 sequence
+// in the reference's draw order (paths outer, draws inner; mc_engine.h:84-136),
+// stored draw-major for the kernels.  Normals come from
+// std::normal_distribution<double>; stratified sampling draws uniforms from
+// std::uniform_real_distribution<double> (stratified_sampling.cpp:23-27).
+// Elements flagged in hb.stream_continues extend the previous element's
+// generator (multilevel_monte_carlo.cpp:20-22: one stream for all levels).
 template <typename T>
 static void fill_streams(const HostBatch& hb, int32_t n, const std::vector<OptionRec>& recs,
                          const std::vector<int64_t>& off, T* buf) {
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<int32_t> starts;
+    for (int32_t i = 0; i < n; ++i)
+        if (!hb.stream_continues || !hb.stream_continues[i] || i == 0) starts.push_back(i);
+    starts.push_back(n);
+    const int32_t groups = int32_t(starts.size()) - 1;
     std::atomic<int32_t> next{0};
     auto work = [&]() {
         for (;;) {
-            const int32_t i = next.fetch_add(1);
-            if (i >= n) break;
-            const OptionRec& rc = recs[i];
-            if (rc.family == FAM_NONE) continue;
-            std::mt19937_64 eng(hb.seed[i]);
+            const int32_t g = next.fetch_add(1);
+            if (g >= groups) break;
+            std::mt19937_64 eng(hb.seed[starts[g]]);
             std::normal_distribution<double> nd(0.0, 1.0);
-            T* base = buf + off[i];
-            if (rc.family == FAM_TERMINAL) {
-                const int64_t cnt = stream_normals(rc);
-                for (int64_t k = 0; k < cnt; ++k) base[k] = T(nd(eng));
-            } else {
-                const int64_t P = stream_units(rc);
-                const int64_t D = stream_draws_per_unit(rc);
-                for (int64_t p = 0; p < P; ++p)
-                    for (int64_t dd = 0; dd < D; ++dd) base[dd * P + p] = T(nd(eng));
+            std::uniform_real_distribution<double> ud(0.0, 1.0);
+            for (int32_t i = starts[g]; i < starts[g + 1]; ++i) {
+                const OptionRec& rc = recs[i];
+                if (rc.family == FAM_NONE) continue;
+                T* base = buf + off[i];
+                if (rc.family == FAM_TERMINAL) {
+                    const int64_t cnt = stream_normals(rc);
+                    if (rc.method == M_STRATIFIED)
+                        for (int64_t k = 0; k < cnt; ++k) base[k] = T(ud(eng));
+                    else
+                        for (int64_t k = 0; k < cnt; ++k) base[k] = T(nd(eng));
+                } else {
+                    const int64_t P = stream_units(rc);
+                    const int64_t D = stream_draws_per_unit(rc);
+                    for (int64_t p = 0; p < P; ++p)
+                        for (int64_t dd = 0; dd < D; ++dd) base[dd * P + p] = T(nd(eng));
+                }
             }
         }
     };

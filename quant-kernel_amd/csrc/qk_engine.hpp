@@ -20,7 +20,10 @@ struct HostBatch {
                  *rho = nullptr;
     const int32_t *ot = nullptr, *paths = nullptr, *steps = nullptr;
     const uint64_t* seed = nullptr;
-    const double* aux = nullptr;  // shift (importance sampling) / weight clip (LR deltas)
+    const double* aux = nullptr;  // shift (importance sampling) / weight clip (LR deltas) / MLMC level
+    // validation modes: element i continues element i-1's host stream (the
+    // levels of one MLMC option draw from one generator, level by level)
+    const uint8_t* stream_continues = nullptr;
 };
 
 // Runs one batch on the GPU.  Returns a QK_* code; on failure *err names the

@@ -71,9 +71,18 @@ struct Quad {
 // Production: Philox blocks keyed by the option seed (wave-uniform key).
 struct PhiloxSrc {
     uint32_t k0, k1;
+    uint32_t tag = 0;  // fourth counter word of path blocks (the MLMC level)
     __device__ __forceinline__ Quad<float> path_block(uint32_t p, uint32_t b, int32_t) const {
-        const Normal4 n = philox_normals(p, b, 0u, k0, k1);
+        const Normal4 n = philox_normals(p, b, tag, k0, k1);
         return Quad<float>{{n.z0, n.z1, n.z2, n.z3}};
+    }
+    // uniforms k = 4g .. 4g+3: (w + 0.5) 2^-32 from block (g, 0, 0, 2)
+    __device__ __forceinline__ void term_uniforms(uint32_t g, double u[4]) const {
+        const U4 w = philox4x32_10(g, 0u, 2u, k0, k1);
+        u[0] = (double(w.x) + 0.5) * 0x1p-32;
+        u[1] = (double(w.y) + 0.5) * 0x1p-32;
+        u[2] = (double(w.z) + 0.5) * 0x1p-32;
+        u[3] = (double(w.w) + 0.5) * 0x1p-32;
     }
     __device__ __forceinline__ Quad<float> term_block(uint32_t g) const {
         const Normal4 n = philox_normals(g, 0u, 1u, k0, k1);
@@ -255,7 +264,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 __device__ __forceinline__ void block_store_partial(double s1, double s2, double extra,
-                                                    Partial* out, double (*red)[3]) {
+                                                    Partial* out, double (*red)[5]) {
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -274,7 +283,38 @@ __device__ __forceinline__ void block_store_partial(double s1, double s2, double
         p.s1 = a;
         p.s2 = b;
         p.extra = extra;
-        p.pad = 0.0;
+        p.x1 = p.x2 = p.xy = 0.0;
+        p.pad[0] = p.pad[1] = 0.0;
+        *out = p;
+    }
+    __syncthreads();
+}
+
+// Control variates: the payoff sums and the control's sums, same order.
+__device__ __forceinline__ void block_store_partial_cv(double s1, double s2, double x1, double x2,
+                                                       double xy, Partial* out,
+                                                       double (*red)[5]) {
+    double v[5] = {s1, s2, x1, x2, xy};
+#pragma unroll
+    for (int i = 0; i < 5; ++i) v[i] = wave_sum(v[i]);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) red[wave][i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int w = 0; w < kBlock / 64; ++w)
+            for (int i = 0; i < 5; ++i) a[i] += red[w][i];
+        Partial p;
+        p.s1 = a[0];
+        p.s2 = a[1];
+        p.extra = 0.0;
+        p.x1 = a[2];
+        p.x2 = a[3];
+        p.xy = a[4];
+        p.pad[0] = p.pad[1] = 0.0;
         *out = p;
     }
     __syncthreads();
@@ -327,6 +367,34 @@ __device__ __forceinline__ R gbm_path(const Src& src, uint32_t p, int32_t steps,
         }
     }
     return s;
+}
+
+// One MLMC correction level (multilevel_monte_carlo.cpp:37-61): per coarse
+// step two fine Euler steps on (z1, z2) and one coarse step of twice the
+// length on (z1 + z2)/sqrt 2, both in the factored form s (c + c' z) with
+// the 1e-12 floor.  steps (fine) is even.
+template <typename R, class Src>
+__device__ __forceinline__ void mlmc_pair_path(const Src& src, uint32_t p, int32_t steps, R s0,
+                                               R c0, R c1, R c2, R c3, R& sf, R& sc) {
+    const R fl = R(1e-12), rs2 = R(0.70710678118654752440);
+    R f = s0, c = s0;
+    auto coarse_step = [&](R z1, R z2) {
+        f = q_max(fl, f * q_fma(c1, z1, c0));
+        f = q_max(fl, f * q_fma(c1, z2, c0));
+        c = q_max(fl, c * q_fma(c3, (z1 + z2) * rs2, c2));
+    };
+    const int32_t nb = steps >> 2;  // one block of 4 draws = two coarse steps
+    for (int32_t b = 0; b < nb; ++b) {
+        const Quad<R> z = src.path_block(p, uint32_t(b), steps);
+        coarse_step(z.z[0], z.z[1]);
+        coarse_step(z.z[2], z.z[3]);
+    }
+    if (steps & 3) {
+        const Quad<R> z = src.path_block(p, uint32_t(nb), steps);
+        coarse_step(z.z[0], z.z[1]);
+    }
+    sf = f;
+    sc = c;
 }
 
 // Heston full-truncation Euler in x = log2(S/spot) (model_concepts.h:205-226):
@@ -554,7 +622,7 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                                                          const R* __restrict__ stream,
                                                          int32_t lane_paths) {
     __shared__ int32_t s_tile;
-    __shared__ double s_red[kBlock / 64][3];
+    __shared__ double s_red[kBlock / 64][5];
     const int32_t n = *n_dev;
     const int32_t total = __builtin_amdgcn_readfirstlane(counters[1]);
     for (;;) {
@@ -608,6 +676,32 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                 s1 += f;
                 s2 = __builtin_fma(f, f, s2);
             }
+        } else if (method == M_MLMC_LEVEL) {
+            // level 0: plain Euler paths; level l >= 1: fine - coarse.  Philox
+            // draws of level l come from blocks (p, d/4, 0, l).
+            const int32_t level = int32_t(rec.aux[0]);
+            const R c0 = R(rec.c[0]), c1 = R(rec.c[1]), c2 = R(rec.c[2]), c3 = R(rec.c[3]);
+            PhiloxSrc pl{rec.seed_lo, rec.seed_hi, uint32_t(level)};
+            for (int32_t j = 0; j < lane_paths; ++j) {
+                const int64_t p = first + threadIdx.x + int64_t(j) * kBlock;
+                if (p >= paths) break;
+                double f;
+                if (level == 0) {
+                    R sp;
+                    if constexpr (MODE == SIM_PHILOX_F32)
+                        sp = gbm_path<R, false>(pl, uint32_t(p), steps, spot, c0, c1, R(0));
+                    else sp = gbm_path<R, false>(ss, uint32_t(p), steps, spot, c0, c1, R(0));
+                    f = double(payoff(sp, strike, ot)) - shift;
+                } else {
+                    R sf, sc;
+                    if constexpr (MODE == SIM_PHILOX_F32)
+                        mlmc_pair_path<R>(pl, uint32_t(p), steps, spot, c0, c1, c2, c3, sf, sc);
+                    else mlmc_pair_path<R>(ss, uint32_t(p), steps, spot, c0, c1, c2, c3, sf, sc);
+                    f = double(payoff(sf, strike, ot)) - double(payoff(sc, strike, ot));
+                }
+                s1 += f;
+                s2 = __builtin_fma(f, f, s2);
+            }
         } else if (method == M_HESTON) {
             const HestonC<R> k = heston_consts<R>(rec);
             HestonPx kp;
@@ -658,7 +752,46 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
 //   TK_PW    pathwise delta: 1[S_T > K] S_T / spot (call), -1[S_T < K] S_T / spot
 //            (put) (pathwise_derivative.cpp:29-35)
 // ----------------------------------------------------------------------------
-enum TermKind : int32_t { TK_PAIR = 0, TK_LR = 1, TK_IS = 2, TK_PW = 3 };
+enum TermKind : int32_t {
+    TK_PAIR = 0, TK_LR = 1, TK_IS = 2, TK_PW = 3,
+    TK_CV = 4,      // control variates: payoff and the control S_T - E[S_T]
+    TK_STRAT = 5,   // stratified: u = (i + U_i) / n
+    TK_SOBOL = 6,   // u = Sobol point i + 1
+    TK_HALTON = 7,  // u = base-2 Halton point i + 1
+};
+
+// Acklam's inverse normal CDF (P. J. Acklam, 2003; relative error < 1.15e-9)
+// in FP64, as monte_carlo_methods/common/internal_util.h:43-83 evaluates it.
+__device__ double acklam(double p) {
+    if (!(p > 0.0 && p < 1.0)) return qnan_d();
+    const double lo = 0.02425, hi = 1.0 - lo;
+    if (p < lo || p > hi) {
+        const double qq = sqrt(-2.0 * log(p < lo ? p : 1.0 - p));
+        const double v = (((((-7.784894002430293e-03 * qq + -3.223964580411365e-01) * qq +
+                             -2.400758277161838e+00) * qq + -2.549732539343734e+00) * qq +
+                           4.374664141464968e+00) * qq + 2.938163982698783e+00) /
+                         ((((7.784695709041462e-03 * qq + 3.224671290700398e-01) * qq +
+                            2.445134137142996e+00) * qq + 3.754408661907416e+00) * qq + 1.0);
+        return p < lo ? v : -v;
+    }
+    const double qq = p - 0.5, rr = qq * qq;
+    return (((((-3.969683028665376e+01 * rr + 2.209460984245205e+02) * rr +
+               -2.759285104469687e+02) * rr + 1.383577518672690e+02) * rr +
+             -3.066479806614716e+01) * rr + 2.506628277459239e+00) * qq /
+           (((((-5.447609879822406e+01 * rr + 1.615858368580409e+02) * rr +
+               -1.556989798598866e+02) * rr + 6.680131188771972e+01) * rr +
+             -1.328068155288572e+01) * rr + 1.0);
+}
+
+// internal_util.h:96-122: the Sobol direction number of bit j is 1 << (31-j),
+// so point i is the bit reverse of gray(i); base-2 Halton is the bit reverse.
+__device__ __forceinline__ double sobol_point(uint32_t i) {
+    return (double(__builtin_bitreverse32(i ^ (i >> 1))) + 0.5) * 0x1p-32;
+}
+__device__ __forceinline__ double halton2_point(uint32_t i) {
+    return double(__builtin_bitreverse32(i)) * 0x1p-32;
+}
+__device__ __forceinline__ double clamp_u(double u) { return fmin(fmax(u, 1e-12), 1.0 - 1e-12); }
 
 template <typename R>
 struct TermC {
@@ -666,6 +799,8 @@ struct TermC {
     R Ah;         // TK_IS: A + B h
     R hn, hc;     // TK_IS: weight exponent -h log2e z - h^2/2 log2e
     R den, inv_den, clip;  // TK_LR
+    R fwd;        // TK_CV: E[S_T]
+    double n;     // TK_STRAT: strata (= paths)
     int32_t ot;
 };
 
@@ -694,21 +829,53 @@ __device__ __forceinline__ double term_unit(const TermC<R>& k, R z) {
     }
 }
 
-// One tile's units: lane handles Philox groups g0 + lane + j*kBlock (4 units each).
+// One tile's units: lane handles Philox groups g0 + lane + j*kBlock (4 units
+// each).  x1, x2, xy accumulate the control (TK_CV only).
 template <typename R, int MODE, int KIND, class PS, class SS>
 __device__ __forceinline__ void term_tile(const TermC<R>& k, const PS& ph, const SS& ss,
                                           int64_t g0, int32_t lane_groups, int32_t units,
-                                          double shift, double& s1, double& s2) {
+                                          double shift, double& s1, double& s2, double& x1,
+                                          double& x2, double& xy) {
+    constexpr bool from_u = KIND == TK_STRAT || KIND == TK_SOBOL || KIND == TK_HALTON;
     for (int32_t j = 0; j < lane_groups; ++j) {
         const int64_t g = g0 + threadIdx.x + int64_t(j) * kBlock;
         if (g * 4 >= units) break;
         Quad<R> z;
-        if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_block(uint32_t(g));
-        else z = ss.term_block(uint32_t(g));
+        double uu[4] = {0.0, 0.0, 0.0, 0.0};
+        if constexpr (KIND == TK_STRAT) {
+            if constexpr (MODE == SIM_PHILOX_F32) {
+                ph.term_uniforms(uint32_t(g), uu);
+            } else {
+                const Quad<R> q = ss.term_block(uint32_t(g));  // the stream holds uniforms
+#pragma unroll
+                for (int c = 0; c < 4; ++c) uu[c] = double(q.z[c]);
+            }
+        } else if constexpr (!from_u) {
+            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_block(uint32_t(g));
+            else z = ss.term_block(uint32_t(g));
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            if (g * 4 + c < units) {
-                const double f = term_unit<R, KIND>(k, z.z[c]) - shift;
+            const int64_t unit = g * 4 + c;
+            if (unit < units) {
+                double f;
+                if constexpr (from_u) {
+                    double u;
+                    if constexpr (KIND == TK_STRAT) u = (double(unit) + uu[c]) / k.n;
+                    else if constexpr (KIND == TK_SOBOL) u = sobol_point(uint32_t(unit + 1));
+                    else u = halton2_point(uint32_t(unit + 1));
+                    const R zr = R(acklam(clamp_u(u)));
+                    f = double(payoff(k.spot * q_exp2(q_fma(k.B, zr, k.A)), k.strike, k.ot)) - shift;
+                } else if constexpr (KIND == TK_CV) {
+                    const R st = k.spot * q_exp2(q_fma(k.B, z.z[c], k.A));
+                    f = double(payoff(st, k.strike, k.ot)) - shift;
+                    const double x = double(st - k.fwd);
+                    x1 += x;
+                    x2 = __builtin_fma(x, x, x2);
+                    xy = __builtin_fma(x, f, xy);
+                } else {
+                    f = term_unit<R, KIND>(k, z.z[c]) - shift;
+                }
                 s1 += f;
                 s2 = __builtin_fma(f, f, s2);
             }
@@ -725,7 +892,7 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
                                                          const R* __restrict__ stream,
                                                          int32_t lane_groups) {
     __shared__ int32_t s_tile;
-    __shared__ double s_red[kBlock / 64][3];
+    __shared__ double s_red[kBlock / 64][5];
     const int32_t n = *n_dev;
     const int32_t total = __builtin_amdgcn_readfirstlane(counters[1]);
     for (;;) {
@@ -749,20 +916,27 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
         k.den = R(rec.aux[2]);
         k.inv_den = R(1.0 / rec.aux[2]);
         k.clip = R(rec.aux[0]);
+        k.fwd = R(rec.aux[0]);
+        k.n = double(paths);
         const double shift = rec.shift;
         const bool anti = method_is_antithetic_terminal(method);
         PhiloxSrc ph{rec.seed_lo, rec.seed_hi};
         StreamSrc<R> ss{stream + rec.stream_off, 1, int64_t(units) + (anti ? (paths & 1) : 0)};
 
-        double s1 = 0.0, s2 = 0.0;
-        if (method == M_LR)
-            term_tile<R, MODE, TK_LR>(k, ph, ss, g0, lane_groups, units, shift, s1, s2);
-        else if (method == M_IMPORTANCE)
-            term_tile<R, MODE, TK_IS>(k, ph, ss, g0, lane_groups, units, shift, s1, s2);
-        else if (method == M_PATHWISE)
-            term_tile<R, MODE, TK_PW>(k, ph, ss, g0, lane_groups, units, shift, s1, s2);
-        else
-            term_tile<R, MODE, TK_PAIR>(k, ph, ss, g0, lane_groups, units, shift, s1, s2);
+        double s1 = 0.0, s2 = 0.0, x1 = 0.0, x2 = 0.0, xy = 0.0;
+#define QK_TERM_TILE(KIND) \
+    term_tile<R, MODE, KIND>(k, ph, ss, g0, lane_groups, units, shift, s1, s2, x1, x2, xy)
+        switch (method) {
+            case M_LR: QK_TERM_TILE(TK_LR); break;
+            case M_IMPORTANCE: QK_TERM_TILE(TK_IS); break;
+            case M_PATHWISE: QK_TERM_TILE(TK_PW); break;
+            case M_CONTROL: QK_TERM_TILE(TK_CV); break;
+            case M_STRATIFIED: QK_TERM_TILE(TK_STRAT); break;
+            case M_SOBOL: QK_TERM_TILE(TK_SOBOL); break;
+            case M_HALTON: QK_TERM_TILE(TK_HALTON); break;
+            default: QK_TERM_TILE(TK_PAIR); break;
+        }
+#undef QK_TERM_TILE
         // antithetic estimators: an odd path count adds one path with a fresh draw
         double extra = 0.0;
         if (anti && (paths & 1) && local_tile == rec.ntiles - 1 && threadIdx.x == 0) {
@@ -773,13 +947,26 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
             extra = method == M_LR ? double(pay * lr_score(z, k.den, k.inv_den, k.clip))
                                    : double(pay);
         }
-        block_store_partial(s1, s2, extra, partials + tile, s_red);
+        if (method == M_CONTROL)
+            block_store_partial_cv(s1, s2, x1, x2, xy, partials + tile, s_red);
+        else
+            block_store_partial(s1, s2, extra, partials + tile, s_red);
     }
 }
 
 // ----------------------------------------------------------------------------
 // finalize
 // ----------------------------------------------------------------------------
+struct Neumaier {  // compensated FP64 sum (Neumaier 1974)
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double x) {
+        const double t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ double sum() const { return s + c; }
+};
+
 __global__ __launch_bounds__(256) void k_finalize(const int32_t* __restrict__ n_dev,
                                                   const OptionRec* __restrict__ recs,
                                                   const Partial* __restrict__ partials,
@@ -801,18 +988,40 @@ __global__ __launch_bounds__(256) void k_finalize(const int32_t* __restrict__ n_
             continue;
         }
         // Neumaier compensated sums over the option's tiles, in tile order.
-        double a = 0.0, ca = 0.0, b = 0.0, cb = 0.0, extra = 0.0;
+        const bool cv = rec.method == M_CONTROL;
+        Neumaier a, b, xa, xb, xc;
+        double extra = 0.0;
         for (int32_t t = 0; t < rec.ntiles; ++t) {
-            const Partial p = partials[rec.tile_first + t];
-            double s = a + p.s1;
-            ca += fabs(a) >= fabs(p.s1) ? (a - s) + p.s1 : (p.s1 - s) + a;
-            a = s;
-            s = b + p.s2;
-            cb += fabs(b) >= fabs(p.s2) ? (b - s) + p.s2 : (p.s2 - s) + b;
-            b = s;
+            const Partial& p = partials[rec.tile_first + t];
+            a.add(p.s1);
+            b.add(p.s2);
             extra += p.extra;
+            if (cv) {
+                xa.add(p.x1);
+                xb.add(p.x2);
+                xc.add(p.xy);
+            }
         }
-        const double s1 = a + ca, s2 = b + cb;
+        const double s1 = a.sum(), s2 = b.sum();
+        if (cv) {
+            // control_variates.cpp:42-53 on centred sums: y = payoff - shift,
+            // x = S_T - E[S_T]; the reference's x and y carry the discount,
+            // so its var_x <= 1e-12 test is disc^2 var_x here
+            const double n = double(rec.units);
+            const double my = s1 / n, mx = xa.sum() / n;
+            const double var_x = xb.sum() / n - mx * mx;
+            const double var_y = s2 / n - my * my;
+            double est = rec.shift + my, res = var_y;
+            if (rec.disc * rec.disc * var_x > kEps) {
+                const double cov = xc.sum() / n - mx * my;
+                const double beta = cov / var_x;
+                est -= beta * mx;
+                res = var_y - cov * beta;
+            }
+            out_price[i] = rec.disc * est;
+            out_se[i] = rec.disc * sqrt(fmax(res, 0.0) / (n - 1.0));
+            continue;
+        }
         const double units = double(rec.units);
         const double mean_d = s1 / units;
         double var = units > 1.0 ? (s2 - s1 * mean_d) / (units - 1.0) : 0.0;
@@ -843,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_finalize(const int32_t* __restrict__ n_
             price = rec.disc * (rec.shift + mean_d);
         }
         out_price[i] = price;
-        out_se[i] = se;
+        out_se[i] = method_is_qmc(m) ? qnan_d() : se;  // no sampling error for point sets
     }
 }
 

@@ -118,6 +118,21 @@ _SIGNATURES = {
     "qk_agm_likelihood_ratio_delta": (_D, [_D] * 6 + [_I, _I, _U, _D]),
     "qk_agm_likelihood_ratio_delta_batch": (_I, [_PD] * 6 + [_PI, _PI, _PU, _PD, _I, _PD]),
     "qk_agm_likelihood_ratio_delta_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PU, _PD, _I, _PD, _PD]),
+    "qk_mcm_control_variates_price": (_D, [_D] * 6 + [_I, _I, _U]),
+    "qk_mcm_control_variates_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_control_variates_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_mcm_stratified_sampling_price": (_D, [_D] * 6 + [_I, _I, _U]),
+    "qk_mcm_stratified_sampling_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_stratified_sampling_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_mcm_quasi_monte_carlo_sobol_price": (_D, [_D] * 6 + [_I, _I]),
+    "qk_mcm_quasi_monte_carlo_sobol_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _I, _PD]),
+    "qk_mcm_quasi_monte_carlo_sobol_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _I, _PD, _PD]),
+    "qk_mcm_quasi_monte_carlo_halton_price": (_D, [_D] * 6 + [_I, _I]),
+    "qk_mcm_quasi_monte_carlo_halton_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _I, _PD]),
+    "qk_mcm_quasi_monte_carlo_halton_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _I, _PD, _PD]),
+    "qk_mcm_multilevel_monte_carlo_price": (_D, [_D] * 6 + [_I, _I, _I, _I, _U]),
+    "qk_mcm_multilevel_monte_carlo_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PI, _PU, _I, _PD]),
+    "qk_mcm_multilevel_monte_carlo_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PI, _PU, _I, _PD, _PD]),
     "qk_mcm_mixed_price_batch": (_I, [_PI] + [_PD] * 11 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
     "qk_mcm_mixed_price_batch_device": (_I, [ct.c_void_p] * 16 + [_I, ct.c_void_p, ct.c_void_p]),
     "qk_mcm_standard_monte_carlo_price_batch_device": (_I, [ct.c_void_p] * 9 + [_I, ct.c_void_p, ct.c_void_p]),
@@ -540,6 +555,76 @@ class QuantKernel:
         return self._call_batch("qk_mcm_heston_monte_carlo_price_batch", *f, ot, p, st, s)
 
     # ---- mixed-model batch (ABI 2.13) ------------------------------------
+    # ---- control variates, stratified sampling, quasi-MC, multilevel MC
+    #      (engine.py:1421-1482, 1535-1647) -----------------------------------
+    def _seeded_batch(self, fn, spot, strike, t, vol, r, q, option_type, paths, seed, se):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        p, s = self._as_i32_array(paths, "paths"), self._as_u64_array(seed, "seed")
+        self._check_same_length(a[0].shape[0], paths=p, seed=s)
+        if se:
+            return self._call_batch(fn + "_se", *a, p, s, with_se=True)
+        return self._call_batch(fn, *a, p, s)
+
+    def _qmc_batch(self, fn, spot, strike, t, vol, r, q, option_type, paths, se):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        p = self._as_i32_array(paths, "paths")
+        self._check_same_length(a[0].shape[0], paths=p)
+        if se:
+            return self._call_batch(fn + "_se", *a, p, with_se=True)
+        return self._call_batch(fn, *a, p)
+
+    def control_variates_price(self, spot, strike, t, vol, r, q, option_type, paths, seed=42):
+        return self._call_checked("qk_mcm_control_variates_price", spot, strike, t, vol, r, q,
+                                  int(option_type), int(paths), int(seed))
+
+    def control_variates_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, seed,
+                                     return_stderr=False):
+        return self._seeded_batch("qk_mcm_control_variates_price_batch", spot, strike, t, vol, r, q,
+                                  option_type, paths, seed, return_stderr)
+
+    def stratified_sampling_price(self, spot, strike, t, vol, r, q, option_type, paths, seed=42):
+        return self._call_checked("qk_mcm_stratified_sampling_price", spot, strike, t, vol, r, q,
+                                  int(option_type), int(paths), int(seed))
+
+    def stratified_sampling_price_batch(self, spot, strike, t, vol, r, q, option_type, paths, seed,
+                                        return_stderr=False):
+        return self._seeded_batch("qk_mcm_stratified_sampling_price_batch", spot, strike, t, vol, r, q,
+                                  option_type, paths, seed, return_stderr)
+
+    def quasi_monte_carlo_sobol_price(self, spot, strike, t, vol, r, q, option_type, paths):
+        return self._call_checked("qk_mcm_quasi_monte_carlo_sobol_price", spot, strike, t, vol, r, q,
+                                  int(option_type), int(paths))
+
+    def quasi_monte_carlo_sobol_price_batch(self, spot, strike, t, vol, r, q, option_type, paths,
+                                            return_stderr=False):
+        return self._qmc_batch("qk_mcm_quasi_monte_carlo_sobol_price_batch", spot, strike, t, vol, r, q,
+                               option_type, paths, return_stderr)
+
+    def quasi_monte_carlo_halton_price(self, spot, strike, t, vol, r, q, option_type, paths):
+        return self._call_checked("qk_mcm_quasi_monte_carlo_halton_price", spot, strike, t, vol, r, q,
+                                  int(option_type), int(paths))
+
+    def quasi_monte_carlo_halton_price_batch(self, spot, strike, t, vol, r, q, option_type, paths,
+                                             return_stderr=False):
+        return self._qmc_batch("qk_mcm_quasi_monte_carlo_halton_price_batch", spot, strike, t, vol, r, q,
+                               option_type, paths, return_stderr)
+
+    def multilevel_monte_carlo_price(self, spot, strike, t, vol, r, q, option_type, base_paths, levels,
+                                     base_steps, seed=42):
+        return self._call_checked("qk_mcm_multilevel_monte_carlo_price", spot, strike, t, vol, r, q,
+                                  int(option_type), int(base_paths), int(levels), int(base_steps), int(seed))
+
+    def multilevel_monte_carlo_price_batch(self, spot, strike, t, vol, r, q, option_type, base_paths, levels,
+                                           base_steps, seed, return_stderr=False):
+        a = self._gbm_arrays(spot, strike, t, vol, r, q, option_type)
+        bp, lv, bs, s = (self._as_i32_array(base_paths, "base_paths"), self._as_i32_array(levels, "levels"),
+                         self._as_i32_array(base_steps, "base_steps"), self._as_u64_array(seed, "seed"))
+        self._check_same_length(a[0].shape[0], base_paths=bp, levels=lv, base_steps=bs, seed=s)
+        fn = "qk_mcm_multilevel_monte_carlo_price_batch"
+        if return_stderr:
+            return self._call_batch(fn + "_se", *a, bp, lv, bs, s, with_se=True)
+        return self._call_batch(fn, *a, bp, lv, bs, s)
+
     # ---- importance sampling and Greeks on the MC engine (engine.py:965-982,
     #      1448-1455, 1583-1598, 1720-1775, 2090-2121) ------------------------
     def importance_sampling_price(self, spot, strike, t, vol, r, q, option_type, paths, shift, seed=42):

@@ -40,6 +40,17 @@ _ORC = {
     "orc_px_pathwise_delta": (D, [D] * 6 + [I, I, U, PD]),
     "orc_px_lr_delta": (D, [D] * 6 + [I, I, U, D, PD]),
     "orc_px_heston_lr_delta": (D, [D] * 10 + [I, I, I, U, D, PD]),
+    "orc_control_variates": (D, [D] * 6 + [I, I, U]),
+    "orc_stratified": (D, [D] * 6 + [I, I, U]),
+    "orc_mt_uniforms": (None, [U, ct.c_int64, PD]),
+    "orc_acklam": (D, [D]),
+    "orc_sobol_point": (D, [ct.c_uint32]),
+    "orc_halton2_point": (D, [ct.c_uint32]),
+    "orc_qmc": (D, [I] + [D] * 6 + [I, I]),
+    "orc_mlmc": (D, [D] * 6 + [I, I, I, I, U]),
+    "orc_px_control_variates": (D, [D] * 6 + [I, I, U, PD]),
+    "orc_px_stratified": (D, [D] * 6 + [I, I, U, PD]),
+    "orc_px_mlmc": (D, [D] * 6 + [I, I, I, I, U, PD]),
     "orc_heston_batch": (None, [PD] * 10 + [PI, PI, PI, PU, I, PD]),
     "orc_standard_mc_batch": (None, [PD] * 6 + [PI, PI, PU, I, PD]),
 }
@@ -97,6 +108,11 @@ _REF = {
     "qk_agm_pathwise_derivative_delta": (D, [D] * 6 + [I, I, U]),
     "qk_agm_likelihood_ratio_delta": (D, [D] * 6 + [I, I, U, D]),
     "qk_mcm_heston_lr_delta": (D, [D] * 10 + [I, I, I, U, D]),
+    "qk_mcm_control_variates_price": (D, [D] * 6 + [I, I, U]),
+    "qk_mcm_stratified_sampling_price": (D, [D] * 6 + [I, I, U]),
+    "qk_mcm_quasi_monte_carlo_sobol_price": (D, [D] * 6 + [I, I]),
+    "qk_mcm_quasi_monte_carlo_halton_price": (D, [D] * 6 + [I, I]),
+    "qk_mcm_multilevel_monte_carlo_price": (D, [D] * 6 + [I, I, I, I, U]),
 }
 
 
