@@ -227,6 +227,19 @@ QK_EXPORT int32_t qk_agm_likelihood_ratio_delta_batch(const double* spot, const 
                                                       const double* weight_clip,
                                                       int32_t n, double* out_prices);
 
+/* ---- Heston characteristic-function price (reference qk_api.h:53-55,
+ *      71-76): the closed-form check of the Heston MC path, on the GPU ---- */
+QK_EXPORT double  qk_cf_heston_price_cf(double spot, double strike, double t, double r, double q,
+                                        double v0, double kappa, double theta, double sigma, double rho,
+                                        int32_t option_type, int32_t integration_steps, double integration_limit);
+QK_EXPORT int32_t qk_cf_heston_price_cf_batch(const double* spot, const double* strike,
+                                              const double* t, const double* r, const double* q,
+                                              const double* v0, const double* kappa, const double* theta,
+                                              const double* sigma, const double* rho,
+                                              const int32_t* option_type, const int32_t* integration_steps,
+                                              const double* integration_limit,
+                                              int32_t n, double* out_prices);
+
 /* ======================================================================== *
  * ABI 2.13 additions.  No reference counterpart: new names only.
  * ======================================================================== */

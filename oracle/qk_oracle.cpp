@@ -676,11 +676,16 @@ ORC_EXPORT double orc_bsm_as(double spot, double strike, double t, double vol, d
 ORC_EXPORT double orc_heston_cf(double spot, double strike, double t, double r, double q,
                                 double v0, double kappa, double theta, double sigma, double rho,
                                 int32_t ot, int32_t integration_steps, double limit) {
+    // heston.cpp:10-24: type, finiteness of every input, ranges, then the
+    // t <= eps branch, then the integration controls
     if (!type_ok(ot)) return qnan();
+    for (double x : {spot, strike, t, r, q, v0, kappa, theta, sigma, rho})
+        if (!finite_bits(x)) return qnan();
     if (spot <= 0.0 || strike <= 0.0 || t < 0.0 || v0 < 0.0 || kappa <= 0.0 || theta < 0.0 ||
         sigma <= 0.0 || rho <= -1.0 || rho >= 1.0)
         return qnan();
     if (t <= kEps) return intrinsic(spot, strike, ot);
+    if (integration_steps < 16 || limit <= 1e-6) return qnan();
     using cd = std::complex<double>;
     const cd I(0.0, 1.0);
     const double s2 = sigma * sigma;

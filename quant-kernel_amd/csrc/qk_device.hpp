@@ -103,6 +103,8 @@ struct Device {
         std::vector<uint64_t> key;
         hipGraphExec_t exec = nullptr;
     };
+    DevBuf gl;               // Gauss-Legendre nodes / weights of the Heston CF pricer
+    bool gl_ready = false;
     std::vector<Graph> graphs;
     bool graphs_ok = true;        // capture and instantiation work on this runtime
     bool graphs_enabled = true;   // caller switch (qk_mc_set_graphs)

@@ -7,6 +7,7 @@
 #include "qk_engine.hpp"
 #include "qk_device.hpp"
 #include "qk_lsmc.hpp"
+#include "qk_cf.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -992,6 +993,102 @@ void apply_cached_shapes(Device* d) {
             sh->grid = d->cus * g;
         }
     }
+}
+
+// ----------------------------------------------------------------------------
+// Heston characteristic-function batch
+// ----------------------------------------------------------------------------
+namespace {
+
+// The 64-point Gauss-Legendre rule on [-1, 1]: Newton iteration on the
+// Legendre recurrence from Tricomi's initial guesses; the 32 positive nodes in
+// ascending order, then their weights.
+std::vector<double> gauss_legendre64() {
+    const int n = 64;
+    std::vector<double> gl(64);
+    for (int i = 0; i < n / 2; ++i) {
+        double z = std::cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double pk = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = pk;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = z;
+        for (int k = 2; k <= n; ++k) {
+            const double pk = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = pk;
+        }
+        dp = n * (z * p1 - p0) / (z * z - 1.0);
+        gl[31 - i] = z;
+        gl[32 + 31 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+    return gl;
+}
+
+}  // namespace
+
+int32_t run_heston_cf_batch(const double* spot, const double* strike, const double* t,
+                            const double* r, const double* q, const double* v0,
+                            const double* kappa, const double* theta, const double* sigma,
+                            const double* rho, const int32_t* ot, const int32_t* steps,
+                            const double* limit, int32_t n, double* out, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+    if (!d->gl_ready) {
+        const std::vector<double> gl = gauss_legendre64();
+        QK_HIP(d->gl.ensure(64 * sizeof(double)));
+        QK_HIP(hipMemcpy(d->gl.p, gl.data(), 64 * sizeof(double), hipMemcpyHostToDevice));
+        d->gl_ready = true;
+    }
+    // stage the 11 double and 2 int arrays through the batch packer
+    std::vector<uint64_t> no_seed(static_cast<size_t>(n), 0);
+    HostBatch hb;
+    hb.spot = spot;
+    hb.strike = strike;
+    hb.t = t;
+    hb.r = r;
+    hb.q = q;
+    hb.v0 = v0;
+    hb.kappa = kappa;
+    hb.theta = theta;
+    hb.sigma = sigma;
+    hb.rho = rho;
+    hb.ot = ot;
+    hb.steps = steps;
+    hb.aux = limit;
+    hb.seed = no_seed.data();
+    BatchIn bi{};
+    const int32_t* n_dev = nullptr;
+    int32_t rc = stage_host_batch(d, hb, n, &bi, &n_dev, err);
+    if (rc != QK_OK) return rc;
+    const size_t ns = align64(size_t(n));
+    QK_HIP(d->out.ensure(ns * 8 + 64));
+    QK_HIP(d->pin_out.ensure(ns * 8 + 64));
+    HestonCfIn in{bi.spot, bi.strike, bi.t, bi.r, bi.q, bi.v0, bi.kappa, bi.theta, bi.sigma,
+                  bi.rho, bi.aux, bi.ot, bi.steps};
+    const int32_t grid = int32_t(std::min<int64_t>((int64_t(n) + 3) / 4, int64_t(d->cus) * 16));
+    QK_HIP(hipEventRecord(d->ev[0], d->stream));
+    QK_HIP(hipEventRecord(d->ev[1], d->stream));
+    QK_HIP(launch_heston_cf(d->stream, in, n, static_cast<const double*>(d->gl.p),
+                            static_cast<double*>(d->out.p), grid));
+    QK_HIP(hipEventRecord(d->ev[2], d->stream));
+    QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, size_t(n) * 8, hipMemcpyDeviceToHost,
+                          d->stream));
+    QK_HIP(hipEventRecord(d->ev[3], d->stream));
+    QK_HIP(hipStreamSynchronize(d->stream));
+    record_timing(d);
+    std::memcpy(out, d->pin_out.p, size_t(n) * 8);
+    return QK_OK;
 }
 
 }  // namespace qk

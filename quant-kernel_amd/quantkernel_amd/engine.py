@@ -133,6 +133,8 @@ _SIGNATURES = {
     "qk_mcm_multilevel_monte_carlo_price": (_D, [_D] * 6 + [_I, _I, _I, _I, _U]),
     "qk_mcm_multilevel_monte_carlo_price_batch": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PI, _PU, _I, _PD]),
     "qk_mcm_multilevel_monte_carlo_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PI, _PU, _I, _PD, _PD]),
+    "qk_cf_heston_price_cf": (_D, [_D] * 10 + [_I, _I, _D]),
+    "qk_cf_heston_price_cf_batch": (_I, [_PD] * 10 + [_PI, _PI, _PD, _I, _PD]),
     "qk_mcm_mixed_price_batch": (_I, [_PI] + [_PD] * 11 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
     "qk_mcm_mixed_price_batch_device": (_I, [ct.c_void_p] * 16 + [_I, ct.c_void_p, ct.c_void_p]),
     "qk_mcm_standard_monte_carlo_price_batch_device": (_I, [ct.c_void_p] * 9 + [_I, ct.c_void_p, ct.c_void_p]),
@@ -555,6 +557,26 @@ class QuantKernel:
         return self._call_batch("qk_mcm_heston_monte_carlo_price_batch", *f, ot, p, st, s)
 
     # ---- mixed-model batch (ABI 2.13) ------------------------------------
+    # ---- Heston characteristic-function price (engine.py:362-372, 422-445) --
+    def heston_price_cf(self, spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type,
+                        integration_steps=1024, integration_limit=120.0):
+        return self._call_checked("qk_cf_heston_price_cf", spot, strike, t, r, q, v0, kappa, theta, sigma, rho,
+                                  int(option_type), int(integration_steps), float(integration_limit))
+
+    def heston_price_cf_batch(self, spot, strike, t, r, q, v0, kappa, theta, sigma, rho, option_type,
+                              integration_steps, integration_limit):
+        f = [self._as_f64_array(x, nm) for x, nm in
+             zip((spot, strike, t, r, q, v0, kappa, theta, sigma, rho),
+                 ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho"))]
+        n = f[0].shape[0]
+        ot, st, lim = (self._as_i32_array(option_type, "option_type"),
+                       self._as_i32_array(integration_steps, "integration_steps"),
+                       self._as_f64_array(integration_limit, "integration_limit"))
+        self._check_same_length(n, **{nm: x for nm, x in zip(
+            ("strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho"), f[1:])},
+            option_type=ot, integration_steps=st, integration_limit=lim)
+        return self._call_batch("qk_cf_heston_price_cf_batch", *f, ot, st, lim)
+
     # ---- control variates, stratified sampling, quasi-MC, multilevel MC
     #      (engine.py:1421-1482, 1535-1647) -----------------------------------
     def _seeded_batch(self, fn, spot, strike, t, vol, r, q, option_type, paths, seed, se):

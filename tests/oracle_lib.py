@@ -108,6 +108,7 @@ _REF = {
     "qk_agm_pathwise_derivative_delta": (D, [D] * 6 + [I, I, U]),
     "qk_agm_likelihood_ratio_delta": (D, [D] * 6 + [I, I, U, D]),
     "qk_mcm_heston_lr_delta": (D, [D] * 10 + [I, I, I, U, D]),
+    "qk_cf_heston_price_cf": (D, [D] * 10 + [I, I, D]),
     "qk_mcm_control_variates_price": (D, [D] * 6 + [I, I, U]),
     "qk_mcm_stratified_sampling_price": (D, [D] * 6 + [I, I, U]),
     "qk_mcm_quasi_monte_carlo_sobol_price": (D, [D] * 6 + [I, I]),

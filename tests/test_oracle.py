@@ -232,6 +232,18 @@ def test_heston_cf(orc, ref):
                               if _bind_ref_cf(ref) else c, abs=1e-9)
 
 
+@pytest.mark.parametrize("j,bad", [(0, -1.0), (1, 0.0), (2, -0.5), (2, 0.0), (2, 1e-13), (3, math.inf),
+                                   (4, math.nan), (5, -0.01), (6, 0.0), (7, -0.1), (8, 0.0), (9, -1.0),
+                                   (9, 1.0), (9, 0.99), (0, math.inf)])
+def test_heston_cf_branches_equal_reference_build(orc, ref, j, bad):
+    base = [100.0, 95.0, 1.0, 0.03, 0.01, 0.04, 1.5, 0.05, 0.5, -0.6]
+    base[j] = bad
+    for ot, st, lim in ((0, 1024, 120.0), (1, 64, 50.0), (0, 15, 120.0), (0, 16, 1e-6), (2, 1024, 120.0)):
+        a = orc.lib.orc_heston_cf(*base, ot, st, lim)
+        b = ref.lib.qk_cf_heston_price_cf(*base, ot, st, lim)
+        assert (math.isnan(a) and math.isnan(b)) or abs(a - b) < 1e-9, (base, ot, st, lim, a, b)
+
+
 def _bind_ref_cf(ref):
     import ctypes as ct
     f = ref.lib.qk_cf_heston_price_cf
