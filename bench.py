@@ -21,11 +21,14 @@ Other workloads (reported with the same fields; not the headline line):
   c2  50000 uneven mixed options (GBM Euler / Milstein / Heston / local vol),
       paths log-uniform in [4096, 1048576], steps log-uniform in [16, 512];
   c3  1024 American puts, Longstaff-Schwartz, 100000 paths x 50 dates,
-      basis {1, x, x^2, x^3}, x = S/K.
-c0 and c3 go through the host-memory entry points (their inputs are 64 and
-100 bytes per option); c1, c2 and c4 use the device-resident ones.
+      basis {1, x, x^2, x^3}, x = S/K;
+  cf  1048576 Heston characteristic-function prices (1024 nodes, FP64).
+c3 goes through the host-memory entry point (its inputs are 100 bytes per
+option); the others use device-resident entry points, replayed as launch
+graphs where the path has them.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c0|c1|c2|c3|c4]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c0|c1|c2|c3|c4|cf]
+       python bench.py --calibration-copy MIB   (the known-size kernel for PMC scaling)
 """
 import argparse
 import ctypes as ct
@@ -389,6 +392,66 @@ def bench_c3(args, qk, qka, dev, name, cus):
     return res
 
 
+# FP64 flops of one Heston CF node: two characteristic-function evaluations
+# (about 104 each: complex products, quotients, square root, two exponentials
+# and a logarithm, with each transcendental counted once) and about 20 for
+# the node's weights, cosine / sine and P1 / P2 terms
+FLOPS_PER_CF_NODE = 230
+PEAK_FP64_TFLOPS = 78.6
+
+
+def bench_cf(args, qk, qka, dev, name, cus):
+    """Heston characteristic-function prices of 1,048,576 queries from the C1
+    parameter box (1024 Gauss-Legendre nodes on [1e-8, 120], the reference
+    wrapper's defaults), device-resident."""
+    import torch
+    from quantkernel_amd import workloads as wl
+    n = args.options or 1 << 20
+    d = wl.config1(n, paths=1, steps=1)
+    d["option_type"] = (np.arange(n) % 2).astype(np.int32)
+    d["integration_steps"] = np.full(n, 1024, np.int32)
+    d["integration_limit"] = np.full(n, 120.0)
+    keys = KEYS + ("option_type", "integration_steps", "integration_limit")
+    t = to_device({k: d[k] for k in keys}, dev)
+    ptrs = {k: v.data_ptr() for k, v in t.items()}
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+
+    def call():
+        qk.heston_price_cf_batch_device(n, ptrs, out.data_ptr())
+
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    ks = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call()
+        ks.append(qk.last_timing()[1])
+        log(f"step done: kernel {ks[-1]:.2f} ms")
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    k_ms = float(np.mean(ks))
+    flops = float(n) * 1024 * FLOPS_PER_CF_NODE
+    ach = flops / (k_ms * 1e-3) / 1e12
+    res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp64",
+               config={"workload": f"Heston characteristic-function prices: {n} calls/puts from the C1 parameter "
+                                   "box, 1024 Gauss-Legendre nodes on [1e-8, 120] (FP64; device-resident)",
+                       "options": n, "device": name, "cus": cus},
+               roofline={"bound": "valu", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "qk::k_heston_cf",
+                         "kernel_ms": k_ms, "flops_per_launch": flops})
+    if not args.no_cpu:
+        kind, fn = ref_fn("qk_cf_heston_price_cf_batch")
+        if fn:
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            arrays = [d[k] for k in keys]
+            k, dt = cpu_sample(fn, arrays, n, threads)
+            res["cpu_baseline"] = {"value": k / dt, "unit": "prices/s", "cores": threads, "kind": kind,
+                                   "sample": f"first {k} queries, reference Heston CF batch entry point, "
+                                             f"{threads} threads"}
+    return res
+
+
 def bench_c4(args, qk, qka, dev, name, cus):
     import torch
     from quantkernel_amd import surrogate as sur
@@ -485,7 +548,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c0", "c1", "c2", "c3", "c4"], default=None)
+    ap.add_argument("--workload", choices=["c0", "c1", "c2", "c3", "c4", "cf"], default=None)
     ap.add_argument("--options", type=int, default=0)
     ap.add_argument("--weights", default=str(ROOT / "quant-kernel_amd" / "quantkernel_amd" / "heston_surrogate.qkw"))
     ap.add_argument("--accuracy-queries", type=int, default=1024)
@@ -517,7 +580,8 @@ def main():
     name, cus = qk.device_info()
     dev = torch.device("cuda:0")
     workload = args.workload or ("c4" if Path(args.weights).exists() else "c1")
-    res = {"c0": bench_c0, "c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[workload](
+    res = {"c0": bench_c0, "c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4,
+           "cf": bench_cf}[workload](
         args, qk, qka, dev, name, cus)
     if PMC_TRAFFIC.exists():
         entry = json.loads(PMC_TRAFFIC.read_text()).get(workload)

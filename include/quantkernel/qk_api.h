@@ -363,6 +363,15 @@ QK_EXPORT int32_t qk_mcm_standard_monte_carlo_price_batch_device(const double* s
         const int32_t* option_type, const int32_t* paths, const uint64_t* seed, int32_t n,
         double* out_prices, double* out_stderr);
 
+/* Heston characteristic-function prices on device-resident inputs and
+ * output (pointers of the current HIP device); equal to
+ * qk_cf_heston_price_cf_batch for the same inputs. */
+QK_EXPORT int32_t qk_cf_heston_price_cf_batch_device(const double* spot, const double* strike,
+        const double* t, const double* r, const double* q, const double* v0, const double* kappa,
+        const double* theta, const double* sigma, const double* rho, const int32_t* option_type,
+        const int32_t* integration_steps, const double* integration_limit, int32_t n,
+        double* out_prices);
+
 /* Longstaff-Schwartz with a selectable regression basis (QK_LSMC_BASIS_*)
  * and standard-error output.  basis QK_LSMC_BASIS_REFERENCE reproduces
  * qk_mcm_longstaff_schwartz_price_batch. */

@@ -783,6 +783,27 @@ int32_t qk_cf_heston_price_cf_batch(const double* spot, const double* strike, co
     return rc;
 }
 
+int32_t qk_cf_heston_price_cf_batch_device(const double* spot, const double* strike,
+                                           const double* t, const double* r, const double* q,
+                                           const double* v0, const double* kappa,
+                                           const double* theta, const double* sigma,
+                                           const double* rho, const int32_t* option_type,
+                                           const int32_t* integration_steps,
+                                           const double* integration_limit, int32_t n,
+                                           double* out_prices) {
+    if (!check_ptrs({QK_P(spot), QK_P(strike), QK_P(t), QK_P(r), QK_P(q), QK_P(v0), QK_P(kappa),
+                     QK_P(theta), QK_P(sigma), QK_P(rho), QK_P(option_type),
+                     QK_P(integration_steps), QK_P(integration_limit), QK_P(out_prices)}))
+        return QK_ERR_NULL_PTR;
+    if (!check_n(n)) return QK_ERR_BAD_SIZE;
+    const qk::HestonCfIn in{spot, strike, t,     r,     q,   v0, kappa, theta, sigma, rho,
+                            integration_limit, option_type, integration_steps};
+    std::string err;
+    const int32_t rc = qk::run_heston_cf_batch_device(in, n, out_prices, &err);
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
 double qk_cf_heston_price_cf(double spot, double strike, double t, double r, double q, double v0,
                              double kappa, double theta, double sigma, double rho,
                              int32_t option_type, int32_t integration_steps,

@@ -25,4 +25,8 @@ int32_t run_heston_cf_batch(const double* spot, const double* strike, const doub
                             const double* rho, const int32_t* ot, const int32_t* steps,
                             const double* limit, int32_t n, double* out, std::string* err);
 
+// Same on device-resident inputs and output (pointers of the current device).
+int32_t run_heston_cf_batch_device(const HestonCfIn& in, int32_t n, double* out_dev,
+                                   std::string* err);
+
 }  // namespace qk

@@ -1035,6 +1035,35 @@ std::vector<double> gauss_legendre64() {
 
 }  // namespace
 
+int32_t ensure_gl(Device* d, std::string* err) {
+    if (!d->gl_ready) {
+        const std::vector<double> gl = gauss_legendre64();
+        QK_HIP(d->gl.ensure(64 * sizeof(double)));
+        QK_HIP(hipMemcpy(d->gl.p, gl.data(), 64 * sizeof(double), hipMemcpyHostToDevice));
+        d->gl_ready = true;
+    }
+    return QK_OK;
+}
+
+int32_t run_heston_cf_batch_device(const HestonCfIn& in, int32_t n, double* out_dev,
+                                   std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+    int32_t rc = ensure_gl(d, err);
+    if (rc != QK_OK) return rc;
+    const int32_t grid = int32_t(std::min<int64_t>((int64_t(n) + 3) / 4, int64_t(d->cus) * 16));
+    QK_HIP(hipEventRecord(d->ev[0], d->stream));
+    QK_HIP(hipEventRecord(d->ev[1], d->stream));
+    QK_HIP(launch_heston_cf(d->stream, in, n, static_cast<const double*>(d->gl.p), out_dev, grid));
+    QK_HIP(hipEventRecord(d->ev[2], d->stream));
+    QK_HIP(hipEventRecord(d->ev[3], d->stream));
+    QK_HIP(hipStreamSynchronize(d->stream));
+    record_timing(d);
+    return QK_OK;
+}
+
 int32_t run_heston_cf_batch(const double* spot, const double* strike, const double* t,
                             const double* r, const double* q, const double* v0,
                             const double* kappa, const double* theta, const double* sigma,
@@ -1044,12 +1073,8 @@ int32_t run_heston_cf_batch(const double* spot, const double* strike, const doub
     if (!d) return QK_ERR_RUNTIME_INIT;
     std::lock_guard<std::mutex> lock(d->mu);
     QK_HIP(hipSetDevice(d->dev));
-    if (!d->gl_ready) {
-        const std::vector<double> gl = gauss_legendre64();
-        QK_HIP(d->gl.ensure(64 * sizeof(double)));
-        QK_HIP(hipMemcpy(d->gl.p, gl.data(), 64 * sizeof(double), hipMemcpyHostToDevice));
-        d->gl_ready = true;
-    }
+    int32_t rc = ensure_gl(d, err);
+    if (rc != QK_OK) return rc;
     // stage the 11 double and 2 int arrays through the batch packer
     std::vector<uint64_t> no_seed(static_cast<size_t>(n), 0);
     HostBatch hb;
@@ -1069,7 +1094,7 @@ int32_t run_heston_cf_batch(const double* spot, const double* strike, const doub
     hb.seed = no_seed.data();
     BatchIn bi{};
     const int32_t* n_dev = nullptr;
-    int32_t rc = stage_host_batch(d, hb, n, &bi, &n_dev, err);
+    rc = stage_host_batch(d, hb, n, &bi, &n_dev, err);
     if (rc != QK_OK) return rc;
     const size_t ns = align64(size_t(n));
     QK_HIP(d->out.ensure(ns * 8 + 64));

@@ -135,6 +135,7 @@ _SIGNATURES = {
     "qk_mcm_multilevel_monte_carlo_price_batch_se": (_I, [_PD] * 6 + [_PI, _PI, _PI, _PI, _PU, _I, _PD, _PD]),
     "qk_cf_heston_price_cf": (_D, [_D] * 10 + [_I, _I, _D]),
     "qk_cf_heston_price_cf_batch": (_I, [_PD] * 10 + [_PI, _PI, _PD, _I, _PD]),
+    "qk_cf_heston_price_cf_batch_device": (_I, [ct.c_void_p] * 13 + [_I, ct.c_void_p]),
     "qk_mcm_mixed_price_batch": (_I, [_PI] + [_PD] * 11 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
     "qk_mcm_mixed_price_batch_device": (_I, [ct.c_void_p] * 16 + [_I, ct.c_void_p, ct.c_void_p]),
     "qk_mcm_standard_monte_carlo_price_batch_device": (_I, [ct.c_void_p] * 9 + [_I, ct.c_void_p, ct.c_void_p]),
@@ -576,6 +577,14 @@ class QuantKernel:
             ("strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho"), f[1:])},
             option_type=ot, integration_steps=st, integration_limit=lim)
         return self._call_batch("qk_cf_heston_price_cf_batch", *f, ot, st, lim)
+
+    def heston_price_cf_batch_device(self, n: int, ptrs: dict, out_price_ptr: int) -> None:
+        """Device-resident Heston CF batch: ``ptrs`` maps argument names to device addresses."""
+        order = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho", "option_type",
+                 "integration_steps", "integration_limit")
+        self._check(self._lib.qk_cf_heston_price_cf_batch_device(
+            *[ct.c_void_p(int(ptrs[k])) for k in order], int(n), ct.c_void_p(int(out_price_ptr))),
+            "qk_cf_heston_price_cf_batch_device")
 
     # ---- control variates, stratified sampling, quasi-MC, multilevel MC
     #      (engine.py:1421-1482, 1535-1647) -----------------------------------

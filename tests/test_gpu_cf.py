@@ -87,3 +87,15 @@ def test_throughput_large_batch(qk):
     total, kernel = qk.last_timing()
     print(f"{n} Heston CF prices (1024 nodes): kernel {kernel:.2f} ms -> {n / kernel * 1e3:.3e} prices/s")
     assert np.all(np.isfinite(got)) and kernel > 0
+
+
+def test_device_entry_equals_host_entry(qk):
+    from hip_mem import DevArray, to_device
+    n = 300
+    d = _batch(n)
+    d["integration_steps"], d["integration_limit"] = d["steps"], d["limit"]
+    host = qk.heston_price_cf_batch(*[d[k] for k in HST], d["option_type"], d["steps"], d["limit"])
+    t = to_device(d, HST + ("option_type", "integration_steps", "integration_limit"))
+    out = DevArray(n=n)
+    qk.heston_price_cf_batch_device(n, {k: v.ptr for k, v in t.items()}, out.ptr)
+    assert np.array_equal(out.numpy(), host, equal_nan=True)
