@@ -254,6 +254,11 @@ double scalar_of(int32_t rc, double v) { return rc == QK_OK ? v : qk::qnan_d(); 
 
 }  // namespace
 
+namespace qk {
+// The thread-local last error, for entry points defined in other files.
+void set_last_error(const char* msg) { set_msg(msg); }
+}  // namespace qk
+
 extern "C" {
 
 // ---- ABI negotiation and errors (reference qk_api.cpp:94-117) -------------
