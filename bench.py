@@ -72,9 +72,9 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(d, budget_s=12.0):
+def cpu_baseline(d, budget_s=12.0, noun="options", note=""):
     """Times the reference build (or, without it, the oracle port) on the
-    first k queries, full paths and steps each."""
+    first k options of d, full paths and steps each."""
     ref = ROOT / "oracle" / "_ref" / "libqkref.so"
     if ref.exists():
         kind, fn = "reference", ct.CDLL(str(ref)).qk_mcm_heston_monte_carlo_price_batch
@@ -101,9 +101,9 @@ def cpu_baseline(d, budget_s=12.0):
         dt = run(k)
         log(f"cpu baseline: {k} options in {dt:.2f}s")
     return {"value": k / dt, "unit": "prices/s", "cores": threads, "kind": kind,
-            "sample": f"first {k} queries of the batch priced by the reference's Heston MC batch entry point "
+            "sample": f"first {k} {noun} of the batch priced by the reference's Heston MC batch entry point "
                       f"({MC_PATHS} paths x {STEPS} steps each, OpenMP over options, {threads} threads, "
-                      f"{cpu_model()}); the reference has no surrogate"}
+                      f"{cpu_model()}){note}"}
 
 
 def to_device(d, dev):
@@ -539,7 +539,7 @@ def bench_c4(args, qk, qka, dev, name, cus):
                          "kernel": "qk::k_sim_stepwise<float, PHILOX> (MC fallback + spot checks)",
                          "kernel_ms": mc_ms_avg, "flops_per_launch": mc_flops})
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(d)
+        res["cpu_baseline"] = cpu_baseline(d, noun="queries", note="; the reference has no surrogate")
     return res
 
 
