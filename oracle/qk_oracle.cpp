@@ -131,7 +131,7 @@ void philox_normals4(uint32_t c0, uint32_t c1, uint32_t c3, uint64_t seed, doubl
     box_muller(w[2], w[3], &z[2], &z[3]);
 }
 
-// Draw d of path p of an option: block (p, d/4), tag 0, component d%4.
+// Draw d of path p of an LSMC option: block (p, d/4), tag 0, component d%4.
 double philox_path_normal(uint64_t seed, uint32_t p, uint32_t d) {
     double z[4];
     philox_normals4(p, d >> 2, 0u, seed, z);
@@ -159,6 +159,16 @@ void philox_heston_normals(uint64_t seed, uint32_t p, uint32_t j, double* z1, do
     const double ang = 6.283185307179586476925286766559 * f;
     *z1 = rad * std::cos(ang);
     *z2 = rad * std::sin(ang);
+}
+
+// GBM / local-vol stepwise (production) stream: step j of path p takes
+// component j%2 of the Heston pair of step j/2, i.e. slot (j%6)/2 of block
+// (p, j/6, 0, 0), the cosine for even j and the sine for odd j: six normals
+// per Philox block.
+double philox_gbm_normal(uint64_t seed, uint32_t p, uint32_t j) {
+    double z1, z2;
+    philox_heston_normals(seed, p, j >> 1, &z1, &z2);
+    return (j & 1u) ? z2 : z1;
 }
 
 // Terminal estimators: draw k: block (k/4, 0), tag 1, component k%4.
@@ -496,6 +506,10 @@ ORC_EXPORT void orc_box_muller(uint32_t a, uint32_t b, double* z0, double* z1) {
     box_muller(a, b, z0, z1);
 }
 
+ORC_EXPORT double orc_philox_gbm_normal(uint64_t seed, uint32_t path, uint32_t step) {
+    return philox_gbm_normal(seed, path, step);
+}
+
 ORC_EXPORT double orc_philox_path_normal(uint64_t seed, uint32_t path, uint32_t draw) {
     return philox_path_normal(seed, path, draw);
 }
@@ -541,7 +555,8 @@ ORC_EXPORT double orc_px_terminal(double spot, double strike, double t, double v
     return disc * (acc / double(paths));
 }
 
-// Stepwise 1-D models: model 0 Euler, 1 Milstein, 3 constant local vol.
+// Stepwise 1-D models: model 0 Euler, 1 Milstein, 3 constant local vol, on the
+// six-normals-per-block stream of philox_gbm_normal.
 ORC_EXPORT double orc_px_stepwise(int32_t model, double spot, double strike, double t, double vol,
                                   double r, double q, int32_t ot, int32_t paths, int32_t steps,
                                   uint64_t seed, double* se) {
@@ -557,7 +572,7 @@ ORC_EXPORT double orc_px_stepwise(int32_t model, double spot, double strike, dou
     for (int32_t i = 0; i < paths; ++i) {
         double s = spot;
         for (int32_t j = 0; j < steps; ++j) {
-            const double dw = sdt * philox_path_normal(seed, uint32_t(i), uint32_t(j));
+            const double dw = sdt * philox_gbm_normal(seed, uint32_t(i), uint32_t(j));
             double nxt = s + mu * s * dt + vol * s * dw;
             if (model == 1) nxt += 0.5 * vol * vol * s * (dw * dw - dt);
             s = std::max(1e-12, nxt);

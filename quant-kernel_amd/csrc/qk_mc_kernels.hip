@@ -503,6 +503,56 @@ __device__ __forceinline__ float heston_path_px(uint32_t p, int32_t steps, const
     return y * k.ck;  // log2(S_T / spot)
 }
 
+// Production GBM Euler / Milstein / local-vol path: the scheme of gbm_path on
+// the Heston stream's pairs, step j taking the cosine (even j) or sine (odd
+// j) of the pair of Heston step j/2, so one Philox block drives six steps.
+// m = -2 ln 2 c1^2 for Euler and local vol folds the volatility into the
+// radius (c1 z = sqrt(m log2 u1) cos theta); Milstein keeps z (m = -2 ln 2).
+template <bool MIL>
+__device__ __forceinline__ void gbm_px_pair(float& s, float u1, float rev, float c0, float c1,
+                                            float c2, float m, bool both) {
+    const float fl = 1e-12f;
+    const float rad = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u1) * m);
+    const float cs = __builtin_amdgcn_cosf(rev);
+    if constexpr (MIL) {
+        const float z = rad * cs;
+        s = q_max(fl, s * q_fma(q_fma(c2, z, c1), z, c0));
+    } else {
+        s = q_max(fl, s * q_fma(rad, cs, c0));
+    }
+    if (both) {
+        const float sn = __builtin_amdgcn_sinf(rev);
+        if constexpr (MIL) {
+            const float z = rad * sn;
+            s = q_max(fl, s * q_fma(q_fma(c2, z, c1), z, c0));
+        } else {
+            s = q_max(fl, s * q_fma(rad, sn, c0));
+        }
+    }
+}
+
+template <bool MIL>
+__device__ __forceinline__ float gbm_path_px(uint32_t p, int32_t steps, float s, float c0, float c1,
+                                             float c2, uint32_t k0, uint32_t k1) {
+    const float m = MIL ? -1.3862943611198906f : -1.3862943611198906f * c1 * c1;
+    const int32_t nb = steps / 6, rem = steps - 6 * nb;
+    for (int32_t b = 0; b < nb; ++b) {
+        const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
+        gbm_px_pair<MIL>(s, u23(w.x), rev23(w.w), c0, c1, c2, m, true);
+        gbm_px_pair<MIL>(s, u23(w.y), rev23(top_bytes(w.x, w.y)), c0, c1, c2, m, true);
+        gbm_px_pair<MIL>(s, u23(w.z), rev23(top_bytes(w.z, w.w)), c0, c1, c2, m, true);
+    }
+    if (rem) {
+        const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
+        gbm_px_pair<MIL>(s, u23(w.x), rev23(w.w), c0, c1, c2, m, rem > 1);
+        if (rem > 2)
+            gbm_px_pair<MIL>(s, u23(w.y), rev23(top_bytes(w.x, w.y)), c0, c1, c2, m, rem > 3);
+        if (rem > 4)
+            gbm_px_pair<MIL>(s, u23(w.z), rev23(top_bytes(w.z, w.w)), c0, c1, c2, m, false);
+    }
+    return s;
+}
+
 // Antithetic Heston pair of heston_lr_delta.cpp:67-96: the forward path and
 // the path driven by the same draws negated; z1f is the forward path's first
 // z1 (the likelihood-ratio score's draw).
@@ -635,7 +685,6 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
         const R spot = R(rec.spot), strike = R(rec.strike);
         const double shift = rec.shift;
 
-        PhiloxSrc ph{rec.seed_lo, rec.seed_hi};
         StreamSrc<R> ss{stream + rec.stream_off, paths,
                         int64_t(steps) * (method == M_HESTON ? 2 : 1)};
 
@@ -725,8 +774,10 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                 if (p >= paths) break;
                 R s;
                 if constexpr (MODE == SIM_PHILOX_F32) {
-                    s = mil ? gbm_path<R, true>(ph, uint32_t(p), steps, spot, c0, c1, c2)
-                            : gbm_path<R, false>(ph, uint32_t(p), steps, spot, c0, c1, c2);
+                    s = mil ? gbm_path_px<true>(uint32_t(p), steps, spot, c0, c1, c2, rec.seed_lo,
+                                                rec.seed_hi)
+                            : gbm_path_px<false>(uint32_t(p), steps, spot, c0, c1, c2, rec.seed_lo,
+                                                 rec.seed_hi);
                 } else {
                     s = mil ? gbm_path<R, true>(ss, uint32_t(p), steps, spot, c0, c1, c2)
                             : gbm_path<R, false>(ss, uint32_t(p), steps, spot, c0, c1, c2);

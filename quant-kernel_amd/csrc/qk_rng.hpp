@@ -2,13 +2,15 @@
 //
 // Production stream: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed
 // by the option's 64-bit seed, followed by Box-Muller (1958) in FP32 on the
-// hardware transcendental units.  Draw d of path p of an option is component
-// d%4 of block (p, d/4, 0, 0); draw k of a terminal (antithetic) estimator is
-// component k%4 of block (k/4, 0, 0, 1).  Words (w0,w1) give components 0,1
-// and (w2,w3) components 2,3 of one block:
+// hardware transcendental units.  Draw d of path p of an LSMC or MLMC path
+// is component d%4 of block (p, d/4, 0, level); draw k of a terminal
+// (antithetic) estimator is component k%4 of block (k/4, 0, 0, 1).  Words
+// (w0,w1) give components 0,1 and (w2,w3) components 2,3 of one block:
 //   u1 = fma(float(w_even), 2^-32, 2^-33)           (FP32, in (0, 1])
 //   f  = (w_odd & 0x7FFFFF) / 2^23                   (exact, in [0, 1))
 //   (z_even, z_odd) = sqrt(-2 ln u1) * (cos 2 pi f, sin 2 pi f)
+// Heston and stepwise GBM paths use three pairs per block instead (u1 from 23
+// bits, angles from the remaining bits), built in the path kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>

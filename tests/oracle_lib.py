@@ -22,6 +22,7 @@ _ORC = {
     "orc_philox4x32_10": (None, [U32P, U32P, U32P]),
     "orc_box_muller": (None, [ct.c_uint32, ct.c_uint32, PD, PD]),
     "orc_philox_path_normal": (D, [U, ct.c_uint32, ct.c_uint32]),
+    "orc_philox_gbm_normal": (D, [U, ct.c_uint32, ct.c_uint32]),
     "orc_philox_terminal_normal": (D, [U, ct.c_uint32]),
     "orc_philox_heston_normals": (None, [U, ct.c_uint32, ct.c_uint32, PD, PD]),
     "orc_px_terminal": (D, [D] * 6 + [I, I, U, I, PD]),

@@ -177,6 +177,42 @@ def test_heston_stream_distribution(orc):
     assert np.abs(z).max() <= math.sqrt(48 * math.log(2)) + 1e-12
 
 
+def test_gbm_stream_is_heston_pairs(orc):
+    """GBM / local-vol stepwise step j: cosine (even j) or sine (odd j) of the
+    Heston pair of step j/2, so six normals per Philox block (p, j/6, 0, 0)."""
+    seed = 0x0BAD_5EED_1234_5678
+    for p in (0, 3, 2**31 + 7):
+        for j in range(14):
+            want = orc.heston_normals(seed, p, j // 2)[j % 2]
+            assert orc.lib.orc_philox_gbm_normal(seed, p, j) == want, (p, j)
+    # steps 0-5 come from block (p, 0) and step 6 from block (p, 1): restated
+    # from the Philox words for slot 0 of block 1
+    w = orc.philox([5, 1, 0, 0], [seed & 0xFFFFFFFF, seed >> 32])
+    u1 = (2.0 * (w[0] & 0x7FFFFF) + 1.0) / 2.0**24
+    ang = 2 * math.pi * (w[3] & 0x7FFFFF) / 2.0**23
+    rad = math.sqrt(-2.0 * math.log(u1))
+    assert orc.lib.orc_philox_gbm_normal(seed, 5, 6) == pytest.approx(rad * math.cos(ang), rel=1e-13, abs=1e-13)
+    assert orc.lib.orc_philox_gbm_normal(seed, 5, 7) == pytest.approx(rad * math.sin(ang), rel=1e-13, abs=1e-13)
+
+
+def test_gbm_stream_distribution(orc):
+    n_paths, n_steps = 4000, 12
+    z = np.array([[orc.lib.orc_philox_gbm_normal(4242, p, j) for j in range(n_steps)] for p in range(n_paths)])
+    zz = z.ravel()
+    assert abs(zz.mean()) < 5 / math.sqrt(zz.size)
+    assert abs(zz.var() - 1) < 5 * math.sqrt(2 / zz.size)
+    k = ((zz - zz.mean()) ** 4).mean() / zz.var() ** 2
+    assert abs(k - 3) < 0.15
+    zs = np.sort(zz)
+    cdf = 0.5 * np.array([math.erfc(-x / math.sqrt(2)) for x in zs])
+    assert np.max(np.abs(cdf - np.arange(1, zs.size + 1) / zs.size)) < 1.63 / math.sqrt(zs.size) * 1.5
+    # consecutive steps (the two halves of one pair, and pairs of one block)
+    # and steps of different blocks are uncorrelated, also in their squares
+    for a, b in ((0, 1), (1, 2), (0, 2), (4, 5), (5, 6), (0, 6)):
+        assert abs(np.corrcoef(z[:, a], z[:, b])[0, 1]) < 5 / math.sqrt(n_paths)
+        assert abs(np.corrcoef(z[:, a] ** 2, z[:, b] ** 2)[0, 1]) < 5 / math.sqrt(n_paths)
+
+
 def test_bsm_known_values_and_parity(orc):
     c = orc.lib.orc_bsm(100, 100, 1, 0.2, 0.05, 0.0, 0)
     p = orc.lib.orc_bsm(100, 100, 1, 0.2, 0.05, 0.0, 1)
