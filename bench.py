@@ -106,6 +106,51 @@ def cpu_baseline(d, budget_s=12.0, noun="options", note=""):
                       f"{cpu_model()}){note}"}
 
 
+def bsm_price(s, k, t, vol, r, q, ot):
+    """Black-Scholes-Merton closed form (Black & Scholes 1973, Merton 1973)."""
+    from scipy.special import ndtr
+    st = vol * np.sqrt(t)
+    d1 = (np.log(s / k) + (r - q + 0.5 * vol * vol) * t) / st
+    d2 = d1 - st
+    call = s * np.exp(-q * t) * ndtr(d1) - k * np.exp(-r * t) * ndtr(d2)
+    return np.where(ot == 0, call, call - s * np.exp(-q * t) + k * np.exp(-r * t))
+
+
+def z_stats(price, se, ref, spot, against):
+    """Errors of MC prices against reference prices, in standard errors and
+    per unit spot.  z moments use the options whose SE is meaningful: with
+    (price / SE)^2 >= 30, about 30 or more paths finish in the money (deep
+    out-of-the-money options with a handful of hits have SE ~ price)."""
+    ok = np.isfinite(price) & np.isfinite(ref) & (se > 0)
+    z = (price[ok] - ref[ok]) / se[ok]
+    e = np.abs(price[ok] - ref[ok]) / spot[ok]
+    rel = (price[ok] / se[ok]) ** 2 >= 30
+    zr = z[rel]
+    return {"against": against, "options": int(ok.sum()), "frac_within_3se": float((np.abs(z) < 3).mean()),
+            "z_options": int(rel.sum()), "mean_z": float(zr.mean()), "rms_z": float(np.sqrt((zr * zr).mean())),
+            "median_abs_err_over_spot": float(np.median(e)), "max_abs_err_over_spot": float(e.max())}
+
+
+def heston_cf_prices(qk, d, idx):
+    """The library's FP64 Heston characteristic-function prices (1024
+    Gauss-Legendre nodes on [1e-8, 120]) of the options idx of d."""
+    m = len(idx)
+    return qk.heston_price_cf_batch(*[d[k][idx] for k in KEYS], d["option_type"][idx],
+                                    np.full(m, 1024, np.int32), np.full(m, 120.0))
+
+
+def oracle_lib():
+    """The FP64 oracle (a checker: used only inside the CPU-baseline legs)."""
+    path = ROOT / "oracle" / "liboracle.so"
+    return ct.CDLL(str(path)) if path.exists() else None
+
+
+def threaded(fn, items, threads):
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(fn, items))
+
+
 def to_device(d, dev):
     import torch
     t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in d.items()}
@@ -188,8 +233,31 @@ def bench_c1(args, qk, qka, dev, name, cus):
                          "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
                          "kernel": "qk::k_sim_stepwise<float, PHILOX>", "kernel_ms": sim_ms,
                          "flops_per_launch": flops})
+    price, sev = out.cpu().numpy(), se.cpu().numpy()
+    idx = np.arange(min(n, 4096))
+    res["config"]["accuracy"] = z_stats(price[idx], sev[idx], heston_cf_prices(qk, d, idx), d["spot"][idx],
+                                        "Heston characteristic-function prices (FP64, this library); the "
+                                        "difference includes the 256-step Euler bias")
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(d)
+        orc = oracle_lib()
+        if orc:
+            f = orc.orc_px_heston
+            f.restype = ct.c_double
+            f.argtypes = [ct.c_double] * 10 + [ct.c_int32] * 3 + [ct.c_uint64, ct.POINTER(ct.c_double)]
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            k = min(n, threads)
+
+            def one(i):
+                e = ct.c_double()
+                return f(*[float(d[key][i]) for key in KEYS], int(d["option_type"][i]), MC_PATHS, STEPS,
+                         int(d["seed"][i]), ct.byref(e))
+
+            want = np.array(threaded(one, range(k), threads))
+            res["cpu_baseline"]["fp64_oracle_check"] = dict(
+                z_stats(price[:k], sev[:k], want, d["spot"][:k],
+                        "FP64 oracle on the same Philox stream (first options of the batch)"),
+                max_abs_err_over_stderr=float(np.max(np.abs(price[:k] - want) / sev[:k])))
     return res
 
 
@@ -265,7 +333,10 @@ def bench_c0(args, qk, qka, dev, name, cus):
                                    "per batch", "options": n,
                        "device_ms": float(np.mean(tot)), "device": name, "cus": cus,
                        "host_entry_prices_per_s_incl_pcie": host_rate,
-                       "median_stderr_over_spot": float(np.median(se / d["spot"]))},
+                       "median_stderr_over_spot": float(np.median(se / d["spot"])),
+                       "accuracy": z_stats(out.cpu().numpy(), se, bsm_price(*[d[k] for k in (
+                           "spot", "strike", "t", "vol", "r", "q", "option_type")]), d["spot"],
+                           "Black-Scholes-Merton closed form")},
                roofline={"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
                          "kernel": "qk::k_sim_terminal<float, PHILOX>", "kernel_ms": sim_ms, "flops_per_launch": flops})
@@ -310,7 +381,18 @@ def bench_c2(args, qk, qka, dev, name, cus):
     ps = d["paths"].astype(np.float64) * d["steps"]
     flops = float(sum(ps[d["model"] == m].sum() * f for m, f in per_step.items()))
     ach = flops / (sim_ms * 1e-3) / 1e12
-    sev = se.cpu().numpy() / d["spot"]
+    price, sea = out.cpu().numpy(), se.cpu().numpy()
+    sev = sea / d["spot"]
+    acc = {}
+    for m, label in ((0, "gbm_euler"), (1, "gbm_milstein"), (3, "local_vol")):
+        idx = np.nonzero(d["model"] == m)[0]
+        acc[label] = z_stats(price[idx], sea[idx], bsm_price(*[d[k][idx] for k in (
+            "spot", "strike", "t", "vol", "r", "q", "option_type")]), d["spot"][idx],
+            "Black-Scholes-Merton closed form; includes the time-discretisation bias")
+    idx = np.nonzero(d["model"] == 2)[0][:4096]
+    acc["heston"] = z_stats(price[idx], sea[idx], heston_cf_prices(qk, d, idx), d["spot"][idx],
+                            "Heston characteristic-function prices (FP64, this library); includes the "
+                            "time-discretisation bias")
     res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
                config={"workload": f"C2: {n} uneven options, 25% each GBM-Euler / GBM-Milstein / Heston-Euler / "
                                    "constant local vol; paths log-uniform [4096, 1048576], steps log-uniform "
@@ -318,7 +400,8 @@ def bench_c2(args, qk, qka, dev, name, cus):
                                    "entry point)",
                        "options": n, "path_steps": float(ps.sum()), "device": name, "cus": cus,
                        "autotuned_shape": {"paths_per_lane": lane, "workgroups_per_cu": grid},
-                       "median_stderr_over_spot": float(np.median(sev)), "max_stderr_over_spot": float(np.max(sev))},
+                       "median_stderr_over_spot": float(np.median(sev)), "max_stderr_over_spot": float(np.max(sev)),
+                       "accuracy": acc},
                roofline={"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
                          "kernel": "qk::k_sim_stepwise<float, PHILOX>", "kernel_ms": sim_ms, "flops_per_launch": flops})
@@ -389,6 +472,19 @@ def bench_c3(args, qk, qka, dev, name, cus):
             res["cpu_baseline"] = {"value": k / dt, "unit": "prices/s", "cores": threads, "kind": kind,
                                    "sample": f"first {k} options, reference LSMC batch (its {{1,S,S^2}} basis), "
                                              f"{threads} threads"}
+            orc = oracle_lib()
+            if orc:
+                f = orc.orc_crr
+                f.restype = ct.c_double
+                f.argtypes = [ct.c_double] * 6 + [ct.c_int32] * 3
+                m = min(n, 256)
+                keys = ("spot", "strike", "t", "vol", "r", "q")
+                tree = np.array(threaded(lambda i: f(*[float(d[key][i]) for key in keys],
+                                                     int(d["option_type"][i]), 2000, 1), range(m), threads))
+                res["cpu_baseline"]["fp64_oracle_check"] = z_stats(
+                    p[:m], se[:m], tree, d["spot"][:m],
+                    "American CRR binomial tree, 2000 steps (FP64 oracle); LSMC is biased low by its "
+                    "sub-optimal exercise rule and the 50 exercise dates")
     return res
 
 
