@@ -301,6 +301,9 @@ def bench_c0(args, qk, qka, dev, name, cus):
     ptrs = {k: v.data_ptr() for k, v in t.items()}
     out = torch.empty(n, dtype=torch.float64, device=dev)
     seo = torch.empty(n, dtype=torch.float64, device=dev)
+    # tile shape from the autotuner (timed on a C0-shaped synthetic batch)
+    lane, grid, tune_ms = qk.autotune(qk.FAMILY_TERMINAL)
+    log(f"autotuned terminal shape: {lane} Philox groups/lane, {grid} workgroups/CU")
 
     def call():
         qk.standard_monte_carlo_price_batch_device(n, ptrs, out.data_ptr(), seo.data_ptr())
@@ -332,6 +335,7 @@ def bench_c0(args, qk, qka, dev, name, cus):
                                    "(Philox, FP32 paths, FP64 sums); device-resident entry point, one launch graph "
                                    "per batch", "options": n,
                        "device_ms": float(np.mean(tot)), "device": name, "cus": cus,
+                       "autotuned_shape": {"philox_groups_per_lane": lane, "workgroups_per_cu": grid},
                        "host_entry_prices_per_s_incl_pcie": host_rate,
                        "median_stderr_over_spot": float(np.median(se / d["spot"])),
                        "accuracy": z_stats(out.cpu().numpy(), se, bsm_price(*[d[k] for k in (

@@ -333,6 +333,10 @@ __device__ __forceinline__ int32_t find_option(const int32_t* __restrict__ tile_
     return lo;
 }
 
+// Persistent workgroups pull tiles from counters[0].  Every workgroup takes
+// one index past the last tile it runs, so the counter can pass 2^31 - 1
+// when the tile count is close to it: callers compare tiles as unsigned, and
+// a wrapped index reads as past the end.  The scan of each launch resets it.
 __device__ __forceinline__ int32_t next_tile(int32_t* counters, int32_t* slot) {
     if (threadIdx.x == 0) *slot = atomicAdd(&counters[0], 1);
     __syncthreads();
@@ -677,7 +681,7 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
     const int32_t total = __builtin_amdgcn_readfirstlane(counters[1]);
     for (;;) {
         const int32_t tile = next_tile(counters, &s_tile);
-        if (tile >= total) break;
+        if (uint32_t(tile) >= uint32_t(total)) break;
         const int32_t opt = find_option(tile_start, n, tile);
         const OptionRec& rec = recs[opt];
         const int32_t method = rec.method, steps = rec.steps, paths = rec.paths, ot = rec.ot;
@@ -948,7 +952,7 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
     const int32_t total = __builtin_amdgcn_readfirstlane(counters[1]);
     for (;;) {
         const int32_t tile = next_tile(counters, &s_tile);
-        if (tile >= total) break;
+        if (uint32_t(tile) >= uint32_t(total)) break;
         const int32_t opt = find_option(tile_start, n, tile);
         const OptionRec& rec = recs[opt];
         const int32_t units = rec.units, paths = rec.paths, method = rec.method;

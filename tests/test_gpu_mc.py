@@ -346,6 +346,26 @@ def test_autotune_and_shapes(qk):
     assert np.array_equal(qk.mixed_price_batch(*a)[0], base)
 
 
+def test_terminal_shapes_change_only_summation_order(qk):
+    """Terminal tiles of 3, 7 and 32 Philox groups per lane (tiles that do
+    and do not divide an option's pairs) give the same prices."""
+    from quantkernel_amd import workloads as wl
+    fam = qk.FAMILY_TERMINAL
+    saved = qk.get_shape(fam)
+    d = wl.config0(40, paths=100000)
+    d["paths"][::7] = 99999  # odd counts: the extra path of the last tile
+    a = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths")]
+    base, se = qk.standard_monte_carlo_price_batch(*a, d["seed"], return_stderr=True)
+    try:
+        for lane in (3, 7, 32):
+            qk.set_shape(fam, lane, 8)
+            got, gse = qk.standard_monte_carlo_price_batch(*a, d["seed"], return_stderr=True)
+            assert np.allclose(got, base, rtol=1e-12, atol=1e-12), lane
+            assert np.allclose(gse, se, rtol=1e-9, atol=1e-15), lane
+    finally:
+        qk.set_shape(fam, *saved)
+
+
 def _to_dev(d, keys):
     from hip_mem import to_device
     return to_device(d, keys)
