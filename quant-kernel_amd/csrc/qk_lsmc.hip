@@ -293,24 +293,45 @@ __global__ __launch_bounds__(kLsmcBlock) void k_lsmc_sweep(const LsmcRec* __rest
     double acc[kRec];
 #pragma unroll
     for (int i = 0; i < kRec; ++i) acc[i] = 0.0;
-    const int32_t p0 = tile * kLsmcTile + threadIdx.x;
-    for (int k = 0; k < kLsmcPerLane; ++k) {
-        const int32_t p = p0 + k * kLsmcBlock;
-        if (p >= P) break;
-        double c = double(cf[p]);
+    // one path: exercise at j, discount, and the sums of date j - 1
+    auto path = [&](int32_t p, R cv, R s, R sp) {
+        double c = double(cv);
         if (do_ex) {
-            const R s = Sj[p];
             const double ex = payoff_d(s, strike, ot);
             if (ex > 0.0 && ex > continuation<R, NB>(beta, s, strike, inv_k)) c = ex;
         }
         if (j >= 2) {
             c *= disc;
-            const R sp = Sm[p];
             if (payoff_d(sp, strike, ot) > 0.0) accumulate<R, NB>(acc, sp, strike, inv_k, c);
             cf[p] = R(c);
         } else {
             acc[kRec - 2] += c;
             acc[kRec - 1] += c * c;
+        }
+    };
+    const int32_t p0 = tile * kLsmcTile + threadIdx.x;
+    if (p0 + (kLsmcPerLane - 1) * kLsmcBlock < P) {
+        // a full tile: the loads of kAhead paths are issued before the first
+        // of them is used, so more bytes are in flight per wave
+        constexpr int kAhead = 2;
+        for (int k0 = 0; k0 < kLsmcPerLane; k0 += kAhead) {
+            R cv[kAhead], sj[kAhead], sm[kAhead];
+#pragma unroll
+            for (int k = 0; k < kAhead; ++k) {
+                const int32_t p = p0 + (k0 + k) * kLsmcBlock;
+                cv[k] = cf[p];
+                sj[k] = do_ex ? Sj[p] : R(0);
+                sm[k] = j >= 2 ? Sm[p] : R(0);
+            }
+#pragma unroll
+            for (int k = 0; k < kAhead; ++k)
+                path(p0 + (k0 + k) * kLsmcBlock, cv[k], sj[k], sm[k]);
+        }
+    } else {
+        for (int k = 0; k < kLsmcPerLane; ++k) {
+            const int32_t p = p0 + k * kLsmcBlock;
+            if (p >= P) break;
+            path(p, cf[p], do_ex ? Sj[p] : R(0), j >= 2 ? Sm[p] : R(0));
         }
     }
     block_store<NB>(acc, part + (size_t(blockIdx.y) * tiles_max + tile) * kRec, s_red);
