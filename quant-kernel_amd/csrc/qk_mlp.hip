@@ -59,6 +59,104 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const __bf16* __restrict__ A, i
     }
 }
 
+// Fused inference: the four hidden layers and the output layer of 128 rows
+// per workgroup in one launch.  Wave w owns rows 32w .. 32w+31 of the tile
+// and computes all 256 outputs of a layer for them (8 MFMA tiles of 32x32).
+// The layer's activations stay in LDS as BF16 (the A operands of the next
+// layer); the weights of a layer pass through LDS in chunks of 64 inputs,
+// staged once per workgroup for its 128 rows.  The hidden activations equal
+// those of k_gemm_nt + k_act_forward bitwise (same MFMA sequence over K, same
+// bias and GELU); the output dot product is summed in another order.
+constexpr int kInfRows = 128;
+constexpr int kInfKc = 64;
+constexpr int kActStride = kH + 8;  // bf16; the pad spreads rows over the LDS banks
+constexpr int kWStride = kInfKc + 8;
+
+__global__ __launch_bounds__(256) void k_mlp_infer(const __bf16* __restrict__ X,
+                                                   const __bf16* __restrict__ wbf,
+                                                   const float* __restrict__ params,
+                                                   float* __restrict__ y, int M) {
+    __shared__ __attribute__((aligned(16))) __bf16 act[kInfRows * kActStride];
+    __shared__ __attribute__((aligned(16))) __bf16 wch[kH * kWStride];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 31, lk = (lane >> 5) * 8;
+    const int row0 = blockIdx.x * kInfRows + wave * 32;  // this wave's first row
+    const int arow = (wave * 32 + li) * kActStride;
+    float ydot[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) ydot[r] = 0.0f;
+    for (int l = 0; l < kHidden; ++l) {
+        const int K = kK[l];
+        const __bf16* Wt = wbf + kBfWt[l];
+        f32x16 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = f32x16{};
+        for (int k0 = 0; k0 < K; k0 += kInfKc) {
+            const int kc = K - k0 < kInfKc ? K - k0 : kInfKc;
+            // the previous chunk and the previous layer's activations are done
+            __syncthreads();
+            {  // thread n stages Wt[n][k0 .. k0 + kc)
+                const bf16x8* src =
+                    reinterpret_cast<const bf16x8*>(Wt + size_t(threadIdx.x) * K + k0);
+                bf16x8* dst = reinterpret_cast<bf16x8*>(wch + threadIdx.x * kWStride);
+                for (int c = 0; c < kc / 8; ++c) dst[c] = src[c];
+            }
+            __syncthreads();
+            for (int ks = 0; ks < kc; ks += 16) {
+                bf16x8 a;
+                if (l == 0) {
+                    const int r = row0 + li;
+                    a = r < M ? *reinterpret_cast<const bf16x8*>(X + size_t(r) * kInPad + k0 + ks + lk)
+                              : bf16x8{};
+                } else {
+                    a = *reinterpret_cast<const bf16x8*>(act + arow + k0 + ks + lk);
+                }
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const bf16x8 b =
+                        *reinterpret_cast<const bf16x8*>(wch + (t * 32 + li) * kWStride + ks + lk);
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+                }
+            }
+        }
+        // bias + GELU; D[i][j]: lane j + 32((i/4)%2), register (i%4) + 4(i/8)
+        const float* bias = params + kBOff[l];
+        const bool last = l == kHidden - 1;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int j = t * 32 + li;
+            const float bj = bias[j];
+            const float w5 = last ? params[kW5 + j] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float h = gelu(acc[t][r] + bj);
+                if (last) {
+                    ydot[r] = __builtin_fmaf(h, w5, ydot[r]);
+                } else {
+                    const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    act[(wave * 32 + i) * kActStride + j] = __bf16(h);
+                }
+            }
+        }
+    }
+    // row i's dot product: the 32 lanes of one half-wave hold its 256 terms
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        float v = ydot[r];
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        ydot[r] = v;
+    }
+    if (li == 0) {
+        const float b5 = params[kB5];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = row0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (row < M) y[row] = ydot[r] + b5;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_act_forward(const float* __restrict__ Z,
                                                      const float* __restrict__ bias, int M, int N,
                                                      float* __restrict__ Zb, __bf16* __restrict__ H,
@@ -222,6 +320,14 @@ hipError_t gemm_nt(hipStream_t st, const void* A, int lda, const void* B, int ld
     hipLaunchKernelGGL(k_gemm_nt, dim3((M + 63) / 64, (N + 63) / 64), dim3(256), 0, st,
                        static_cast<const __bf16*>(A), lda, static_cast<const __bf16*>(B), ldb, C,
                        ldc, M, N, K);
+    return hipGetLastError();
+}
+
+hipError_t infer(hipStream_t st, const void* X, const void* wbf, const float* params, float* y,
+                 int M) {
+    if (M <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_infer, dim3((M + kInfRows - 1) / kInfRows), dim3(256), 0, st,
+                       static_cast<const __bf16*>(X), static_cast<const __bf16*>(wbf), params, y, M);
     return hipGetLastError();
 }
 

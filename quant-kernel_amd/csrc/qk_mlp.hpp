@@ -71,6 +71,11 @@ __host__ __device__ inline double heston_control(const double* f) {
 // M, N multiples of 32, K a multiple of 16, lda/ldb multiples of 8.
 hipError_t gemm_nt(hipStream_t st, const void* A, int lda, const void* B, int ldb, float* C, int ldc,
                    int M, int N, int K);
+// y[m] = network(X[m]) for m < M, all layers in one launch: X is [M][kInPad]
+// bf16 (rows >= M are neither read nor written), wbf the BF16 forward
+// weights, params the FP32 biases, w5 and b5.
+hipError_t infer(hipStream_t st, const void* X, const void* wbf, const float* params, float* y,
+                 int M);
 // Zb = Z + b; H = gelu(Zb) (bf16, [M][N]); optional Zb (fp32), Ht (bf16 [N][ldt]), Hf (fp32).
 hipError_t act_forward(hipStream_t st, const float* Z, const float* bias, int M, int N, float* Zb,
                        void* H, void* Ht, int ldt, float* Hf);

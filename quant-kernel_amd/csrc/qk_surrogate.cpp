@@ -311,8 +311,8 @@ int32_t sur_predict(const double* feat, int32_t n, double* out, std::string* err
         const int Mb = round_up(Mv, 64);
         QK_HIP(gather(d->stream, P<float>(g_sur.feat), nullptr, nullptr, start, Mv, Mb, g_sur.X.p,
                       nullptr, nullptr));
-        rc = forward(d->stream, Mb, Mv, false, err);
-        if (rc != QK_OK) return rc;
+        QK_HIP(infer(d->stream, g_sur.X.p, g_sur.wbf.p, P<float>(g_sur.params), P<float>(g_sur.y),
+                     Mv));
         QK_HIP(hipMemcpyAsync(hy.data() + start, g_sur.y.p, size_t(Mv) * 4, hipMemcpyDeviceToHost,
                               d->stream));
     }
@@ -472,8 +472,6 @@ int32_t sur_heston_price(const BatchIn& in_arg, int32_t n, bool device, double* 
     in.paths_override = 0;
 
     const int Mpad = round_up(std::max(n, 1), 64);
-    rc = alloc_work(kChunk, false, err);
-    if (rc != QK_OK) return rc;
     rc = ensure_mc_capacity(d, n, err);
     if (rc != QK_OK) return rc;
     QK_HIP(g_sur.sX.ensure(size_t(Mpad) * kInPad * 2));
@@ -498,16 +496,8 @@ int32_t sur_heston_price(const BatchIn& in_arg, int32_t n, bool device, double* 
     QK_HIP(sur_prepare(st, in, n, Mpad, g_sur.spot_every, g_sur.sX.p, P<uint8_t>(g_sur.act_fb),
                        P<uint8_t>(g_sur.act_spot), P<int8_t>(g_sur.status),
                        P<double>(g_sur.closed)));
-    // 2. network, in chunks of kChunk rows
-    for (int start = 0; start < Mpad; start += kChunk) {
-        const int Mb = std::min(kChunk, Mpad - start);
-        QK_HIP(hipMemcpyAsync(g_sur.X.p, static_cast<char*>(g_sur.sX.p) + size_t(start) * kInPad * 2,
-                              size_t(Mb) * kInPad * 2, hipMemcpyDeviceToDevice, st));
-        rc = forward(st, Mb, Mb, false, err);
-        if (rc != QK_OK) return rc;
-        QK_HIP(hipMemcpyAsync(P<float>(g_sur.sy) + start, g_sur.y.p, size_t(Mb) * 4,
-                              hipMemcpyDeviceToDevice, st));
-    }
+    // 2. network: every row in one fused launch
+    QK_HIP(infer(st, g_sur.sX.p, g_sur.wbf.p, P<float>(g_sur.params), P<float>(g_sur.sy), Mpad));
     QK_HIP(sur_price(st, in, n, P<float>(g_sur.sy), P<int8_t>(g_sur.status), P<double>(g_sur.sur)));
     QK_HIP(hipEventRecord(e1, st));
     // 3. spot checks (plain MC on a sample of the answered queries)

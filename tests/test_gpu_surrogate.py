@@ -41,6 +41,21 @@ def test_forward_and_gradients_match_cpu_oracle(qk):
     assert np.all(w1[:, mo.K_IN:] == 0.0)
 
 
+def test_fused_inference_matches_training_forward(qk):
+    """Predictions come from the fused one-launch inference kernel, the loss
+    of sur_gradients from the per-layer training forward: the same network
+    in two implementations (hidden layers bitwise equal, the output dot
+    product summed in another order)."""
+    p = mo.init_params(9)
+    qk.sur_set_params(p)
+    for n in (1, 127, 128, 129, 1000):  # partial and full 128-row tiles
+        f = _box(n, 10 + n)
+        t = np.random.default_rng(n).random(n) * 0.3
+        y = qk.sur_predict(f) - mo.control(f)
+        _, loss = qk.sur_gradients(f, t)
+        assert float(np.mean((y - t) ** 2)) == pytest.approx(loss, rel=1e-5, abs=1e-12), n
+
+
 def test_training_fits_and_is_deterministic(qk):
     f = _box(16384, 7)
     # a smooth price-like target: BSM-style call value per unit spot
