@@ -170,6 +170,28 @@ def test_philox_vs_reference_within_3se(qk, orc):
     assert abs(np.mean(z)) < 3.0 / math.sqrt(24) * 1.5
 
 
+@pytest.mark.parametrize("model", [0, 1, 3])
+def test_philox_stepwise_vs_reference_within_3se(qk, orc, model):
+    """Euler, Milstein and local vol on the six-normals-per-block Philox
+    stream against the same estimator on the reference's mt19937 stream
+    (validation mode, FP64): two independent estimates of one price."""
+    d = wl.config0(24, paths=20000)
+    d["steps"] = np.array([7, 12, 23, 28, 33, 38] * 4, np.int32)  # every step count mod 6
+    args = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths", "steps")]
+    name = {0: "euler_maruyama", 1: "milstein", 3: "local_vol_monte_carlo"}[model]
+    fn = getattr(qk, f"{name}_price_batch")
+    got, se = fn(*args, d["seed"], return_stderr=True)
+    qk.set_rng_mode(RNG_F64)
+    try:
+        ref_like, rse = fn(*args, d["seed"], return_stderr=True)
+    finally:
+        qk.set_rng_mode(RNG_PHILOX)
+    z = (got - ref_like) / np.sqrt(se * se + rse * rse)
+    print(name, "z", np.round(z, 2))
+    assert np.all(np.abs(z) < 4.5)
+    assert abs(np.mean(z)) < 3.0 / math.sqrt(24) * 1.5
+
+
 def test_terminal_vs_bsm_within_3se(qk, orc):
     d = wl.config0(64, paths=200_000)
     args = [d[k] for k in ("spot", "strike", "t", "vol", "r", "q", "option_type", "paths")]
