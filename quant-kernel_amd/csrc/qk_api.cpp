@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <exception>
+#include <new>
 
 #include "qk_cf.hpp"
 #include "qk_engine.hpp"
@@ -32,6 +34,22 @@ void set_msg(const char* m) {
 }
 
 void set_msg(const std::string& m) { set_msg(m.c_str()); }
+
+// Runs one engine call; an exception (a failed host allocation, above all)
+// becomes QK_ERR_RUNTIME_INIT with a message instead of crossing the C ABI.
+template <class F>
+int32_t guard(std::string& err, F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        err = "host memory allocation failed";
+    } catch (const std::exception& e) {
+        err = std::string("internal error: ") + e.what();
+    } catch (...) {
+        err = "internal error";
+    }
+    return QK_ERR_RUNTIME_INIT;
+}
 
 bool check_ptrs(std::initializer_list<std::pair<const void*, const char*>> ptrs) {
     for (const auto& p : ptrs) {
@@ -62,7 +80,9 @@ const QKPluginAPI k_api = {QK_ABI_MAJOR, QK_ABI_MINOR,
 
 int32_t run(const qk::HostBatch& hb, int32_t n, double* out, double* se) {
     std::string err;
-    const int32_t rc = qk::run_mc_batch(hb, n, out, se, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::run_mc_batch(hb, n, out, se, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -174,10 +194,11 @@ int32_t qmc_batch(int32_t method, const double* spot, const double* strike, cons
 // levels <= 0 or base_steps <= 0 give NaN (multilevel_monte_carlo.cpp:14-17);
 // so do level counts whose finest step count overflows int32 (the reference
 // shifts past 2^31 there).
-int32_t mlmc_batch(const double* spot, const double* strike, const double* t, const double* vol,
-                   const double* r, const double* q, const int32_t* ot, const int32_t* base_paths,
-                   const int32_t* levels, const int32_t* base_steps, const uint64_t* seed,
-                   int32_t n, double* out, double* se) {
+int32_t mlmc_batch_impl(const double* spot, const double* strike, const double* t,
+                        const double* vol, const double* r, const double* q, const int32_t* ot,
+                        const int32_t* base_paths, const int32_t* levels,
+                        const int32_t* base_steps, const uint64_t* seed, int32_t n, double* out,
+                        double* se) {
     std::vector<int64_t> first(static_cast<size_t>(n));
     std::vector<int32_t> count(static_cast<size_t>(n));
     int64_t total = 0;
@@ -248,6 +269,20 @@ int32_t mlmc_batch(const double* spot, const double* strike, const double* t, co
         if (se) se[i] = std::sqrt(var);
     }
     return QK_OK;
+}
+
+// mlmc_batch_impl builds its per-level expansion in host vectors.
+int32_t mlmc_batch(const double* spot, const double* strike, const double* t, const double* vol,
+                   const double* r, const double* q, const int32_t* ot, const int32_t* base_paths,
+                   const int32_t* levels, const int32_t* base_steps, const uint64_t* seed,
+                   int32_t n, double* out, double* se) {
+    std::string err;
+    const int32_t rc = guard(err, [&] {
+        return mlmc_batch_impl(spot, strike, t, vol, r, q, ot, base_paths, levels, base_steps,
+                               seed, n, out, se);
+    });
+    if (!err.empty()) set_msg(err);
+    return rc;
 }
 
 double scalar_of(int32_t rc, double v) { return rc == QK_OK ? v : qk::qnan_d(); }
@@ -781,9 +816,11 @@ int32_t qk_cf_heston_price_cf_batch(const double* spot, const double* strike, co
         return QK_ERR_NULL_PTR;
     if (!check_n(n)) return QK_ERR_BAD_SIZE;
     std::string err;
-    const int32_t rc = qk::run_heston_cf_batch(spot, strike, t, r, q, v0, kappa, theta, sigma, rho,
-                                               option_type, integration_steps, integration_limit,
-                                               n, out_prices, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::run_heston_cf_batch(spot, strike, t, r, q, v0, kappa, theta, sigma, rho,
+                                       option_type, integration_steps, integration_limit, n,
+                                       out_prices, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -804,7 +841,9 @@ int32_t qk_cf_heston_price_cf_batch_device(const double* spot, const double* str
     const qk::HestonCfIn in{spot, strike, t,     r,     q,   v0, kappa, theta, sigma, rho,
                             integration_limit, option_type, integration_steps};
     std::string err;
-    const int32_t rc = qk::run_heston_cf_batch_device(in, n, out_prices, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::run_heston_cf_batch_device(in, n, out_prices, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -815,9 +854,11 @@ double qk_cf_heston_price_cf(double spot, double strike, double t, double r, dou
                              double integration_limit) {
     double out = 0.0;
     std::string err;
-    const int32_t rc = qk::run_heston_cf_batch(&spot, &strike, &t, &r, &q, &v0, &kappa, &theta,
-                                               &sigma, &rho, &option_type, &integration_steps,
-                                               &integration_limit, 1, &out, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::run_heston_cf_batch(&spot, &strike, &t, &r, &q, &v0, &kappa, &theta, &sigma,
+                                       &rho, &option_type, &integration_steps, &integration_limit,
+                                       1, &out, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return scalar_of(rc, out);
 }
@@ -889,7 +930,9 @@ int32_t qk_mcm_mixed_price_batch_device(const int32_t* model, const double* spot
     bi.steps = steps;
     bi.seed = seed;
     std::string err;
-    const int32_t rc = qk::run_mc_batch_device(bi, n, out_prices, out_stderr, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::run_mc_batch_device(bi, n, out_prices, out_stderr, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -918,7 +961,9 @@ int32_t qk_mcm_standard_monte_carlo_price_batch_device(const double* spot, const
     bi.paths = paths;
     bi.seed = seed;
     std::string err;
-    const int32_t rc = qk::run_mc_batch_device(bi, n, out_prices, out_stderr, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::run_mc_batch_device(bi, n, out_prices, out_stderr, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -965,21 +1010,27 @@ int32_t qk_mc_philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3,
     if (!check_ptrs({QK_P(out_words), QK_P(out_normals)})) return QK_ERR_NULL_PTR;
     if (!check_n(count)) return QK_ERR_BAD_SIZE;
     std::string err;
-    const int32_t rc = qk::philox_probe(seed, c0, c1, c3, count, out_words, out_normals, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::philox_probe(seed, c0, c1, c3, count, out_words, out_normals, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
 
 int32_t qk_mc_graphs(int32_t enable, int32_t* out_cached, int64_t* out_launches) {
     std::string err;
-    const int32_t rc = qk::set_graphs(enable, out_cached, out_launches, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::set_graphs(enable, out_cached, out_launches, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
 
 int32_t qk_mc_set_shape(int32_t family, int32_t lane_units, int32_t grid_per_cu) {
     std::string err;
-    const int32_t rc = qk::set_shape(family, lane_units, grid_per_cu, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::set_shape(family, lane_units, grid_per_cu, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -987,7 +1038,9 @@ int32_t qk_mc_set_shape(int32_t family, int32_t lane_units, int32_t grid_per_cu)
 int32_t qk_mc_get_shape(int32_t family, int32_t* lane_units, int32_t* grid_per_cu) {
     if (!check_ptrs({QK_P(lane_units), QK_P(grid_per_cu)})) return QK_ERR_NULL_PTR;
     std::string err;
-    const int32_t rc = qk::get_shape(family, lane_units, grid_per_cu, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::get_shape(family, lane_units, grid_per_cu, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -995,7 +1048,9 @@ int32_t qk_mc_get_shape(int32_t family, int32_t* lane_units, int32_t* grid_per_c
 int32_t qk_mc_autotune(int32_t family, int32_t* out_lane_units, int32_t* out_grid_per_cu,
                        double* out_ms) {
     std::string err;
-    const int32_t rc = qk::autotune(family, out_lane_units, out_grid_per_cu, out_ms, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::autotune(family, out_lane_units, out_grid_per_cu, out_ms, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -1034,8 +1089,10 @@ int32_t qk_mcm_longstaff_schwartz_price_batch_ex(const double* spot, const doubl
         return QK_ERR_INVALID_INPUT;
     }
     std::string err;
-    const int32_t rc = qk::run_lsmc_batch(spot, strike, t, vol, r, q, option_type, paths, steps,
-                                          seed, basis, n, out_prices, out_stderr, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::run_lsmc_batch(spot, strike, t, vol, r, q, option_type, paths, steps, seed,
+                                  basis, n, out_prices, out_stderr, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -1051,9 +1108,10 @@ int32_t qk_mcm_longstaff_schwartz_price_batch(const double* spot, const double* 
         return QK_ERR_NULL_PTR;
     if (!check_n(n)) return QK_ERR_BAD_SIZE;
     std::string err;
-    const int32_t rc =
-        qk::run_lsmc_batch(spot, strike, t, vol, r, q, option_type, paths, steps, seed,
-                           QK_LSMC_BASIS_REFERENCE, n, out_prices, nullptr, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::run_lsmc_batch(spot, strike, t, vol, r, q, option_type, paths, steps, seed,
+                                  QK_LSMC_BASIS_REFERENCE, n, out_prices, nullptr, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return rc;
 }
@@ -1063,9 +1121,10 @@ double qk_mcm_longstaff_schwartz_price(double spot, double strike, double t, dou
                                        int32_t steps, uint64_t seed) {
     double out = 0.0;
     std::string err;
-    const int32_t rc = qk::run_lsmc_batch(&spot, &strike, &t, &vol, &r, &q, &option_type, &paths,
-                                          &steps, &seed, QK_LSMC_BASIS_REFERENCE, 1, &out,
-                                          nullptr, &err);
+    const int32_t rc = guard(err, [&] {
+        return qk::run_lsmc_batch(&spot, &strike, &t, &vol, &r, &q, &option_type, &paths, &steps,
+                                  &seed, QK_LSMC_BASIS_REFERENCE, 1, &out, nullptr, &err);
+    });
     if (rc != QK_OK) set_msg(err);
     return scalar_of(rc, out);
 }
@@ -1075,12 +1134,12 @@ double qk_mcm_longstaff_schwartz_price(double spot, double strike, double t, dou
 // ---- Heston surrogate (ABI 2.13) ------------------------------------------
 extern "C" {
 
-#define QK_SUR_RC(expr)                    \
-    do {                                   \
-        std::string err_;                  \
-        const int32_t rc_ = (expr);        \
-        if (rc_ != QK_OK) set_msg(err_);   \
-        return rc_;                        \
+#define QK_SUR_RC(expr)                                         \
+    do {                                                        \
+        std::string err_;                                       \
+        const int32_t rc_ = guard(err_, [&] { return (expr); }); \
+        if (rc_ != QK_OK) set_msg(err_);                        \
+        return rc_;                                             \
     } while (0)
 
 int32_t qk_sur_param_count(void) { return qk::mlp::kParams; }
