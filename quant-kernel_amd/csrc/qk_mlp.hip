@@ -59,16 +59,20 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const __bf16* __restrict__ A, i
     }
 }
 
-// Fused inference: the four hidden layers and the output layer of 128 rows
-// per workgroup in one launch.  Wave w owns rows 32w .. 32w+31 of the tile
-// and computes all 256 outputs of a layer for them (8 MFMA tiles of 32x32).
-// The layer's activations stay in LDS as BF16 (the A operands of the next
-// layer); the weights of a layer pass through LDS in chunks of 64 inputs,
-// staged once per workgroup for its 128 rows.  The hidden activations equal
-// those of k_gemm_nt + k_act_forward bitwise (same MFMA sequence over K, same
-// bias and GELU); the output dot product is summed in another order.
-constexpr int kInfRows = 128;
-constexpr int kInfKc = 64;
+// Fused inference: the four hidden layers and the output layer of 64 rows
+// per workgroup in one launch.  Waves 2r and 2r+1 share rows 32r .. 32r+31
+// of the tile and compute output columns 0-127 and 128-255 of a layer for
+// them (4 MFMA tiles of 32x32 each).  The layer's activations stay in LDS as
+// BF16 (the A operands of the next layer); the weights of a layer pass
+// through LDS in chunks of kInfKc inputs, staged once per workgroup, and
+// each thread loads its row of the next chunk (the next layer's first after
+// a layer's last) into registers while the current chunk's MFMAs run.  At
+// about 55 KB of LDS two workgroups share a CU, so one wave's MFMAs overlap
+// another's GELU epilogue.  The hidden activations equal those of k_gemm_nt
+// + k_act_forward bitwise (same MFMA sequence over K, same bias and GELU);
+// the output dot product is summed in another order.
+constexpr int kInfRows = 64;
+constexpr int kInfKc = 32;  // every layer's K is a multiple of 32
 constexpr int kActStride = kH + 8;  // bf16; the pad spreads rows over the LDS banks
 constexpr int kWStride = kInfKc + 8;
 
@@ -78,31 +82,43 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const __bf16* __restrict__ X,
                                                    float* __restrict__ y, int M) {
     __shared__ __attribute__((aligned(16))) __bf16 act[kInfRows * kActStride];
     __shared__ __attribute__((aligned(16))) __bf16 wch[kH * kWStride];
+    __shared__ float s_half[kInfRows];  // the column-half 1 dot products
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = wave >> 1, wc = wave & 1;  // row block, column half
     const int li = lane & 31, lk = (lane >> 5) * 8;
-    const int row0 = blockIdx.x * kInfRows + wave * 32;  // this wave's first row
-    const int arow = (wave * 32 + li) * kActStride;
+    const int row0 = blockIdx.x * kInfRows + wr * 32;  // the row block's first row
+    const int arow = (wr * 32 + li) * kActStride;
     float ydot[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) ydot[r] = 0.0f;
-    for (int l = 0; l < kHidden; ++l) {
-        const int K = kK[l];
-        const __bf16* Wt = wbf + kBfWt[l];
-        f32x16 acc[8];
+    // thread n's row of weight chunk c of layer l: Wt_l[n][32c .. 32c + 32)
+    bf16x8 pre[kInfKc / 8];
+    auto load_chunk = [&](int l, int c) {
+        const bf16x8* src = reinterpret_cast<const bf16x8*>(
+            wbf + kBfWt[l] + size_t(threadIdx.x) * kK[l] + c * kInfKc);
 #pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = f32x16{};
-        for (int k0 = 0; k0 < K; k0 += kInfKc) {
-            const int kc = K - k0 < kInfKc ? K - k0 : kInfKc;
-            // the previous chunk and the previous layer's activations are done
+        for (int q = 0; q < kInfKc / 8; ++q) pre[q] = src[q];
+    };
+    load_chunk(0, 0);
+    for (int l = 0; l < kHidden; ++l) {
+        const int nch = kK[l] / kInfKc;
+        f32x16 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = f32x16{};
+        for (int c = 0; c < nch; ++c) {
+            // the previous chunk (and, at c = 0, the previous epilogue) done
             __syncthreads();
-            {  // thread n stages Wt[n][k0 .. k0 + kc)
-                const bf16x8* src =
-                    reinterpret_cast<const bf16x8*>(Wt + size_t(threadIdx.x) * K + k0);
+            {
                 bf16x8* dst = reinterpret_cast<bf16x8*>(wch + threadIdx.x * kWStride);
-                for (int c = 0; c < kc / 8; ++c) dst[c] = src[c];
+#pragma unroll
+                for (int q = 0; q < kInfKc / 8; ++q) dst[q] = pre[q];
             }
             __syncthreads();
-            for (int ks = 0; ks < kc; ks += 16) {
+            if (c + 1 < nch) load_chunk(l, c + 1);
+            else if (l + 1 < kHidden) load_chunk(l + 1, 0);
+            const int k0 = c * kInfKc;
+#pragma unroll
+            for (int ks = 0; ks < kInfKc; ks += 16) {
                 bf16x8 a;
                 if (l == 0) {
                     const int r = row0 + li;
@@ -112,19 +128,22 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const __bf16* __restrict__ X,
                     a = *reinterpret_cast<const bf16x8*>(act + arow + k0 + ks + lk);
                 }
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const bf16x8 b =
-                        *reinterpret_cast<const bf16x8*>(wch + (t * 32 + li) * kWStride + ks + lk);
+                for (int t = 0; t < 4; ++t) {
+                    const int n = wc * 128 + t * 32 + li;
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(wch + n * kWStride + ks + lk);
                     acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
                 }
             }
         }
+        // both column halves have read this layer's A operands before either
+        // overwrites them
+        __syncthreads();
         // bias + GELU; D[i][j]: lane j + 32((i/4)%2), register (i%4) + 4(i/8)
         const float* bias = params + kBOff[l];
         const bool last = l == kHidden - 1;
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int j = t * 32 + li;
+        for (int t = 0; t < 4; ++t) {
+            const int j = wc * 128 + t * 32 + li;
             const float bj = bias[j];
             const float w5 = last ? params[kW5 + j] : 0.0f;
 #pragma unroll
@@ -134,12 +153,12 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const __bf16* __restrict__ X,
                     ydot[r] = __builtin_fmaf(h, w5, ydot[r]);
                 } else {
                     const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    act[(wave * 32 + i) * kActStride + j] = __bf16(h);
+                    act[(wr * 32 + i) * kActStride + j] = __bf16(h);
                 }
             }
         }
     }
-    // row i's dot product: the 32 lanes of one half-wave hold its 256 terms
+    // row i's half dot product: the 32 lanes of one half-wave hold its terms
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         float v = ydot[r];
@@ -147,12 +166,18 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const __bf16* __restrict__ X,
         for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
         ydot[r] = v;
     }
-    if (li == 0) {
+    if (wc == 1 && li == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) s_half[wr * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)] = ydot[r];
+    }
+    __syncthreads();
+    if (wc == 0 && li == 0) {
         const float b5 = params[kB5];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int row = row0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (row < M) y[row] = ydot[r] + b5;
+            const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int row = row0 + i;
+            if (row < M) y[row] = ydot[r] + s_half[wr * 32 + i] + b5;
         }
     }
 }
