@@ -28,7 +28,7 @@ using namespace mlp;
 
 namespace {
 
-constexpr int kChunk = 65536;  // rows per inference pass
+constexpr int kChunk = 65536;  // rows per gather + inference pass of sur_predict
 
 struct Sur {
     bool have = false;
@@ -74,29 +74,27 @@ int32_t alloc_params(Device* d, std::string* err) {
     return QK_OK;
 }
 
-// Buffers for batches of M rows (training when train is set).
-int32_t alloc_work(int M, bool train, std::string* err) {
+// Training buffers for batches of M rows.
+int32_t alloc_work(int M, std::string* err) {
     const size_t MH = size_t(M) * kH;
     QK_HIP(g_sur.X.ensure(size_t(M) * kInPad * 2));
     QK_HIP(g_sur.Z.ensure(MH * 4));
     QK_HIP(g_sur.H4f.ensure(MH * 4));
     QK_HIP(g_sur.y.ensure(size_t(M) * 4));
-    for (int l = 0; l < kHidden; ++l) QK_HIP(g_sur.H[l].ensure(MH * 2));
-    if (train) {
-        QK_HIP(g_sur.Xt.ensure(size_t(M) * kInPad * 2));
-        QK_HIP(g_sur.tgt.ensure(size_t(M) * 4));
-        for (int l = 0; l < kHidden; ++l) {
-            QK_HIP(g_sur.Zb[l].ensure(MH * 4));
-            QK_HIP(g_sur.Ht[l].ensure(MH * 2));
-        }
-        QK_HIP(g_sur.dH.ensure(MH * 4));
-        QK_HIP(g_sur.dZf.ensure(MH * 4));
-        QK_HIP(g_sur.dZ.ensure(MH * 2));
-        QK_HIP(g_sur.dZt.ensure(MH * 2));
-        QK_HIP(g_sur.dy.ensure(size_t(M) * 4));
-        QK_HIP(g_sur.sq.ensure(size_t(M) * 4));
-        QK_HIP(g_sur.part.ensure(size_t((M + 127) / 128 + 1) * kH * 4));
+    QK_HIP(g_sur.Xt.ensure(size_t(M) * kInPad * 2));
+    QK_HIP(g_sur.tgt.ensure(size_t(M) * 4));
+    for (int l = 0; l < kHidden; ++l) {
+        QK_HIP(g_sur.H[l].ensure(MH * 2));
+        QK_HIP(g_sur.Zb[l].ensure(MH * 4));
+        QK_HIP(g_sur.Ht[l].ensure(MH * 2));
     }
+    QK_HIP(g_sur.dH.ensure(MH * 4));
+    QK_HIP(g_sur.dZf.ensure(MH * 4));
+    QK_HIP(g_sur.dZ.ensure(MH * 2));
+    QK_HIP(g_sur.dZt.ensure(MH * 2));
+    QK_HIP(g_sur.dy.ensure(size_t(M) * 4));
+    QK_HIP(g_sur.sq.ensure(size_t(M) * 4));
+    QK_HIP(g_sur.part.ensure(size_t((M + 127) / 128 + 1) * kH * 4));
     return QK_OK;
 }
 
@@ -105,25 +103,25 @@ T* P(DevBuf& b) {
     return static_cast<T*>(b.p);
 }
 
-// Forward pass of M rows already in X; y gets the outputs.
-int32_t forward(hipStream_t st, int M, int M_valid, bool train, std::string* err) {
+// Training forward pass of M rows already in X (M_valid of them real): keeps
+// every layer's pre-activations and activations for the backward pass; y
+// gets the outputs, dy and sq the loss terms.  Inference uses the fused
+// kernel (infer) instead.
+int32_t forward(hipStream_t st, int M, int M_valid, std::string* err) {
+    (void)err;
     const float* prm = P<float>(g_sur.params);
     const __bf16* wbf = P<__bf16>(g_sur.wbf);
     const void* in = g_sur.X.p;
     for (int l = 0; l < kHidden; ++l) {
         const int K = kK[l];
         QK_HIP(gemm_nt(st, in, K, wbf + kBfWt[l], K, P<float>(g_sur.Z), kH, M, kH, K));
-        // inference ping-pongs H[0]/H[1]; training keeps every layer for backward
-        void* out = train ? g_sur.H[l].p : g_sur.H[l & 1].p;
-        QK_HIP(act_forward(st, P<float>(g_sur.Z), prm + kBOff[l], M, kH,
-                           train ? P<float>(g_sur.Zb[l]) : nullptr, out,
-                           train ? g_sur.Ht[l].p : nullptr, M,
+        QK_HIP(act_forward(st, P<float>(g_sur.Z), prm + kBOff[l], M, kH, P<float>(g_sur.Zb[l]),
+                           g_sur.H[l].p, g_sur.Ht[l].p, M,
                            l == kHidden - 1 ? P<float>(g_sur.H4f) : nullptr));
-        in = out;
+        in = g_sur.H[l].p;
     }
-    QK_HIP(out_forward(st, P<float>(g_sur.H4f), prm, M, M_valid,
-                       train ? P<float>(g_sur.tgt) : nullptr, P<float>(g_sur.y),
-                       train ? P<float>(g_sur.dy) : nullptr, train ? P<float>(g_sur.sq) : nullptr));
+    QK_HIP(out_forward(st, P<float>(g_sur.H4f), prm, M, M_valid, P<float>(g_sur.tgt),
+                       P<float>(g_sur.y), P<float>(g_sur.dy), P<float>(g_sur.sq)));
     return QK_OK;
 }
 
@@ -236,7 +234,7 @@ int32_t sur_train(const double* feat, const double* target, int32_t n, int32_t e
         return QK_ERR_INVALID_INPUT;
     }
     const int M = round_up(batch, 64);
-    rc = alloc_work(M, true, err);
+    rc = alloc_work(M, err);
     if (rc != QK_OK) return rc;
     std::vector<float> hf(size_t(n) * kIn), ht(static_cast<size_t>(n));
     for (int32_t i = 0; i < n; ++i) {
@@ -268,7 +266,7 @@ int32_t sur_train(const double* feat, const double* target, int32_t n, int32_t e
             QK_HIP(gather(d->stream, P<float>(g_sur.feat), P<float>(g_sur.tgts),
                           P<int32_t>(g_sur.perm), start, Mv, Mb, g_sur.X.p, g_sur.Xt.p,
                           P<float>(g_sur.tgt)));
-            rc = forward(d->stream, Mb, Mv, true, err);
+            rc = forward(d->stream, Mb, Mv, err);
             if (rc != QK_OK) return rc;
             rc = backward(d->stream, Mb, err);
             if (rc != QK_OK) return rc;
@@ -300,8 +298,8 @@ int32_t sur_predict(const double* feat, int32_t n, double* out, std::string* err
     QK_HIP(hipSetDevice(d->dev));
     int32_t rc = need_model(err);
     if (rc != QK_OK) return rc;
-    rc = alloc_work(kChunk, false, err);
-    if (rc != QK_OK) return rc;
+    QK_HIP(g_sur.X.ensure(size_t(kChunk) * kInPad * 2));
+    QK_HIP(g_sur.y.ensure(size_t(kChunk) * 4));
     std::vector<float> hf(size_t(n) * kIn), hy(static_cast<size_t>(n));
     for (int32_t i = 0; i < n; ++i) normalise(feat + size_t(i) * kIn, &hf[size_t(i) * kIn]);
     QK_HIP(g_sur.feat.ensure(hf.size() * 4));
@@ -334,7 +332,7 @@ int32_t sur_gradients(const double* feat, const double* target, int32_t n, float
         return QK_ERR_INVALID_INPUT;
     }
     const int M = round_up(n, 64);
-    rc = alloc_work(M, true, err);
+    rc = alloc_work(M, err);
     if (rc != QK_OK) return rc;
     std::vector<float> hf(size_t(n) * kIn), ht(static_cast<size_t>(n));
     for (int32_t i = 0; i < n; ++i) {
@@ -348,7 +346,7 @@ int32_t sur_gradients(const double* feat, const double* target, int32_t n, float
     QK_HIP(hipMemcpyAsync(g_sur.tgts.p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, d->stream));
     QK_HIP(gather(d->stream, P<float>(g_sur.feat), P<float>(g_sur.tgts), nullptr, 0, n, M,
                   g_sur.X.p, g_sur.Xt.p, P<float>(g_sur.tgt)));
-    rc = forward(d->stream, M, n, true, err);
+    rc = forward(d->stream, M, n, err);
     if (rc != QK_OK) return rc;
     rc = backward(d->stream, M, err);
     if (rc != QK_OK) return rc;
