@@ -1,0 +1,132 @@
+"""Validation mode against the reference build, family by family, on batches.
+
+With the RNG mode QK_RNG_MT19937_FP64 the library simulates in FP64 on the
+reference's own mt19937_64 + normal_distribution stream (mc_engine.h:15-23),
+so every batch entry point must return the reference's prices within 1e-6
+relative (an absolute floor of 1e-9 per unit spot covers prices near zero).
+The reference is the library compiled from its own sources into
+oracle/_ref/libqkref.so; its scalar entry points give the expected values.
+The quasi-MC estimators use no random stream and are compared the same way.
+"""
+import numpy as np
+import pytest
+
+from quantkernel_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+RNG_PHILOX, RNG_F64 = 0, 1
+N = 6
+GBM = ("spot", "strike", "t", "vol", "r", "q")
+HESTON = ("spot", "strike", "t", "r", "q", "v0", "kappa", "theta", "sigma", "rho")
+
+
+def _gbm(paths, steps=None):
+    d = wl.config0(N, paths=paths)
+    d["option_type"] = (np.arange(N) % 2).astype(np.int32)
+    if steps is not None:
+        d["steps"] = np.array([steps, steps + 3, steps + 7, steps, steps + 1, steps + 5][:N], np.int32)
+    return d
+
+
+def _close(got, want, spot):
+    got, want = np.asarray(got, float), np.asarray(want, float)
+    tol = 1e-6 * np.abs(want) + 1e-9 * spot
+    bad = ~(np.abs(got - want) <= tol)
+    assert not bad.any(), list(zip(got[bad], want[bad]))
+
+
+def _f(x, i):
+    return float(x[i]) if x.dtype.kind == "f" else int(x[i])
+
+
+@pytest.fixture
+def validation(qk):
+    qk.set_rng_mode(RNG_F64)
+    yield qk
+    qk.set_rng_mode(RNG_PHILOX)
+
+
+@pytest.mark.parametrize("name", ["standard_monte_carlo", "antithetic_variates", "control_variates",
+                                  "stratified_sampling", "pathwise_derivative_delta"])
+def test_terminal_families(validation, ref, name):
+    qk = validation
+    d = _gbm(4001)
+    args = [d[k] for k in GBM] + [d["option_type"], d["paths"]]
+    suffix = "" if name.endswith("delta") else "_price"
+    got = getattr(qk, f"{name}{suffix}_batch")(*args, d["seed"])
+    prefix = "agm_" if name.endswith("delta") else "mcm_"
+    want = [getattr(ref, f"{prefix}{name}{suffix}")(*[_f(a, i) for a in args], int(d["seed"][i]))
+            for i in range(N)]
+    _close(got, want, d["spot"])
+
+
+@pytest.mark.parametrize("name", ["quasi_monte_carlo_sobol", "quasi_monte_carlo_halton"])
+def test_quasi_mc(validation, ref, name):
+    qk = validation
+    d = _gbm(3001)
+    args = [d[k] for k in GBM] + [d["option_type"], d["paths"]]
+    got = getattr(qk, f"{name}_price_batch")(*args)
+    want = [getattr(ref, f"mcm_{name}_price")(*[_f(a, i) for a in args]) for i in range(N)]
+    _close(got, want, d["spot"])
+
+
+def test_importance_sampling_and_lr_delta(validation, ref):
+    qk = validation
+    d = _gbm(4001)
+    args = [d[k] for k in GBM] + [d["option_type"], d["paths"]]
+    shift = np.array([0.0, 0.3, -0.2, 0.5, 1.0, -0.7])
+    got = qk.importance_sampling_price_batch(*args, shift, d["seed"])
+    want = [ref.mcm_importance_sampling_price(*[_f(a, i) for a in args], float(shift[i]), int(d["seed"][i]))
+            for i in range(N)]
+    _close(got, want, d["spot"])
+    clip = np.full(N, 6.0)
+    got = qk.likelihood_ratio_delta_batch(*args, d["seed"], clip)
+    want = [ref.agm_likelihood_ratio_delta(*[_f(a, i) for a in args], int(d["seed"][i]), 6.0) for i in range(N)]
+    _close(got, want, d["spot"])
+
+
+@pytest.mark.parametrize("name", ["euler_maruyama", "milstein", "local_vol_monte_carlo"])
+def test_stepwise_families(validation, ref, name):
+    qk = validation
+    d = _gbm(2001, steps=24)
+    args = [d[k] for k in GBM] + [d["option_type"], d["paths"], d["steps"]]
+    got = getattr(qk, f"{name}_price_batch")(*args, d["seed"])
+    want = [getattr(ref, f"mcm_{name}_price")(*[_f(a, i) for a in args], int(d["seed"][i])) for i in range(N)]
+    _close(got, want, d["spot"])
+
+
+def test_longstaff_schwartz(validation, ref):
+    qk = validation
+    d = wl.config3(N, paths=3000, steps=20)
+    args = [d[k] for k in GBM] + [d["option_type"], d["paths"], d["steps"]]
+    got = qk.longstaff_schwartz_price_batch(*args, d["seed"])
+    want = [ref.mcm_longstaff_schwartz_price(*[_f(a, i) for a in args], int(d["seed"][i])) for i in range(N)]
+    _close(got, want, d["spot"])
+
+
+def test_multilevel(validation, ref):
+    qk = validation
+    d = _gbm(1)
+    base_paths = np.array([2000, 1500, 3000, 1000, 2500, 1200], np.int32)
+    levels = np.array([1, 2, 3, 4, 2, 3], np.int32)
+    base_steps = np.array([4, 2, 8, 2, 6, 4], np.int32)
+    args = [d[k] for k in GBM] + [d["option_type"]]
+    got = qk.multilevel_monte_carlo_price_batch(*args, base_paths, levels, base_steps, d["seed"])
+    want = [ref.mcm_multilevel_monte_carlo_price(*[_f(a, i) for a in args], int(base_paths[i]), int(levels[i]),
+                                                  int(base_steps[i]), int(d["seed"][i])) for i in range(N)]
+    _close(got, want, d["spot"])
+
+
+def test_heston_and_lr_delta(validation, ref):
+    qk = validation
+    d = wl.config1(N, paths=3001, steps=30)
+    d["option_type"] = (np.arange(N) % 2).astype(np.int32)
+    args = [d[k] for k in HESTON] + [d["option_type"], d["paths"], d["steps"]]
+    got = qk.heston_monte_carlo_price_batch(*args, d["seed"])
+    want = [ref.mcm_heston_monte_carlo_price(*[_f(a, i) for a in args], int(d["seed"][i])) for i in range(N)]
+    _close(got, want, d["spot"])
+    clip = np.full(N, 6.0)
+    got = qk.heston_lr_delta_batch(*args, d["seed"], clip)
+    want = [ref.mcm_heston_lr_delta(*[_f(a, i) for a in args], int(d["seed"][i]), 6.0) for i in range(N)]
+    _close(got, want, d["spot"])
