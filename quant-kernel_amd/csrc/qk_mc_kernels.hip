@@ -254,6 +254,53 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int32_t* __rest
     if (i == n) tile_start[n] = counters[1];
 }
 
+// One-pass scan for batches of at most kScanSmallBlocks blocks: each block
+// sums the tile counts before its own range, and of the whole batch, itself
+// (at most 64 counts per thread), which costs less than the two launches of
+// the three-pass scan it replaces.  Offsets, tile_start[n] and the counters
+// are the three-pass scan's.
+constexpr int kScanSmallBlocks = 64;
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_small(const int32_t* __restrict__ n_dev,
+                                                           const int32_t* __restrict__ ntiles,
+                                                           int32_t* __restrict__ counters,
+                                                           int32_t capacity,
+                                                           OptionRec* __restrict__ recs,
+                                                           int32_t* __restrict__ tile_start) {
+    __shared__ long long sh[kScanBlock];
+    const int32_t n = *n_dev;
+    const int32_t base = blockIdx.x * kScanBlock;
+    long long before = 0, all = 0;
+    for (int32_t k = threadIdx.x; k < n; k += kScanBlock) {
+        const long long v = ntiles[k];
+        all += v;
+        if (k < base) before += v;
+    }
+    block_inclusive_scan(before, sh);
+    before = sh[kScanBlock - 1];
+    __syncthreads();
+    block_inclusive_scan(all, sh);
+    all = sh[kScanBlock - 1];
+    __syncthreads();
+    const bool too_big = all > 0x7fffffffLL;
+    const bool over = too_big || all > capacity;
+    const int32_t i = base + threadIdx.x;
+    const long long v = i < n ? ntiles[i] : 0;
+    const long long inc = block_inclusive_scan(v, sh);
+    if (i < n) {
+        const int32_t off = over ? 0 : int32_t(before + inc - v);
+        tile_start[i] = off;
+        recs[i].tile_first = off;
+    }
+    if (i == n) tile_start[n] = over ? 0 : int32_t(all);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[0] = 0;
+        counters[1] = over ? 0 : int32_t(all);
+        counters[2] = too_big ? 2 : (over ? 1 : 0);
+        counters[3] = too_big ? 0x7fffffff : int32_t(all);
+    }
+}
+
 // ----------------------------------------------------------------------------
 // block reduction of (s1, s2, extra) in FP64, fixed order
 // ----------------------------------------------------------------------------
@@ -1158,6 +1205,11 @@ hipError_t launch_scan(hipStream_t st, int32_t n, const int32_t* n_dev, const in
                        int32_t capacity) {
     // n + 1 elements so that the thread of index n writes tile_start[n]
     const int32_t nb = (n + 1 + kScanBlock - 1) / kScanBlock;
+    if (nb <= kScanSmallBlocks) {
+        hipLaunchKernelGGL(k_scan_small, dim3(nb), dim3(kScanBlock), 0, st, n_dev, ntiles, counters,
+                           capacity, recs, tile_start);
+        return hipGetLastError();
+    }
     long long* bs = static_cast<long long*>(scratch);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, st, n_dev, ntiles, bs);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, st, nb, bs, counters, capacity);
