@@ -27,14 +27,15 @@ __device__ __forceinline__ Cx cmul(Cx a, Cx b) {
 }
 __device__ __forceinline__ Cx cscale(Cx a, double s) { return {a.re * s, a.im * s}; }
 
-// Smith's division (scaled by the larger denominator component).
+// Smith's division (scaled by the larger denominator component), with one
+// reciprocal of the scaled denominator in place of two divisions.
 __device__ __forceinline__ Cx cdiv(Cx a, Cx b) {
     if (fabs(b.re) >= fabs(b.im)) {
-        const double r = b.im / b.re, den = b.re + b.im * r;
-        return {(a.re + a.im * r) / den, (a.im - a.re * r) / den};
+        const double r = b.im / b.re, inv = 1.0 / (b.re + b.im * r);
+        return {(a.re + a.im * r) * inv, (a.im - a.re * r) * inv};
     }
-    const double r = b.re / b.im, den = b.re * r + b.im;
-    return {(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+    const double r = b.re / b.im, inv = 1.0 / (b.re * r + b.im);
+    return {(a.re * r + a.im) * inv, (a.im * r - a.re) * inv};
 }
 
 // Principal square root (branch cut on the negative real axis, the sign of
@@ -63,8 +64,9 @@ struct HestonCf {
     double v0, kappa, rho_sigma, sigma2, inv_sigma2, kt_inv_s2, drift, t;
 };
 
-// heston.cpp:33-44: phi(u) = E[exp(i u ln S_T)].
-__device__ Cx heston_phi(const HestonCf& h, Cx u) {
+// heston.cpp:33-44: phi(u) = E[exp(i u ln S_T)], times exp(-i rot) for a real
+// rot folded into the final exponential (the integrands' e^{-i u ln K}).
+__device__ Cx heston_phi(const HestonCf& h, Cx u, double rot) {
     const Cx iu = {-u.im, u.re};
     const Cx beta = {h.kappa - h.rho_sigma * iu.re, -h.rho_sigma * iu.im};
     const Cx d = csqrt(cadd(cmul(beta, beta), cscale(cadd(iu, cmul(u, u)), h.sigma2)));
@@ -76,7 +78,8 @@ __device__ Cx heston_phi(const HestonCf& h, Cx u) {
     const Cx C = cadd(cscale(iu, h.drift),
                       cscale(csub(cscale(bmd, h.t), cscale(lg, 2.0)), h.kt_inv_s2));
     const Cx D = cmul(cscale(bmd, h.inv_sigma2), cdiv({1.0 - e.re, -e.im}, one_m_ge));
-    return cexp(cadd(C, cscale(D, h.v0)));
+    const Cx x = cadd(C, cscale(D, h.v0));
+    return cexp({x.re, x.im - rot});
 }
 
 __device__ __forceinline__ double wave_sum64(double v) {
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(kCfBlock) void k_heston_cf(HestonCfIn in, int32_t n
             h.t = t;
             const double log_k = log(k);
             // internal_util.h:83-95: phi(-i) normalises P1; panels of 64 nodes
-            const Cx phi_mi = heston_phi(h, {0.0, -1.0});
+            const Cx phi_mi = heston_phi(h, {0.0, -1.0}, 0.0);
             const double phi_abs = hypot(phi_mi.re, phi_mi.im);
             const bool p1_ok = phi_abs >= 1e-14;
             const double phi_abs2 = phi_mi.re * phi_mi.re + phi_mi.im * phi_mi.im;
@@ -133,16 +136,13 @@ __global__ __launch_bounds__(kCfBlock) void k_heston_cf(HestonCfIn in, int32_t n
                 const double pa = a + double(p) * width, pb = pa + width;
                 const double mid = 0.5 * (pa + pb), half = 0.5 * (pb - pa);
                 const double u = side == 0 ? mid - half * xi : mid + half * xi;
-                const double inv_u = 1.0 / u;
-                const Cx cf2 = heston_phi(h, {u, 0.0});
-                double sn, cs;
-                sincos(u * log_k, &sn, &cs);
-                s2 += half * wi * ((cs * cf2.im - sn * cf2.re) * inv_u);
+                const double inv_u = 1.0 / u, rot = u * log_k;
+                // e^{-i u ln K} phi(u) and e^{-i u ln K} phi(u - i)
+                const Cx cf2 = heston_phi(h, {u, 0.0}, rot);
+                s2 += half * wi * (cf2.im * inv_u);
                 if (p1_ok) {
-                    const Cx cf1 = heston_phi(h, {u, -1.0});
-                    const double n1_re = cs * cf1.re + sn * cf1.im;
-                    const double n1_im = cs * cf1.im - sn * cf1.re;
-                    const double d_re = n1_im * inv_u, d_im = -n1_re * inv_u;
+                    const Cx n1 = heston_phi(h, {u, -1.0}, rot);
+                    const double d_re = n1.im * inv_u, d_im = -n1.re * inv_u;
                     s1 += half * wi * ((d_re * phi_mi.re + d_im * phi_mi.im) / phi_abs2);
                 }
             }
