@@ -902,6 +902,7 @@ struct TermC {
     R hn, hc;     // TK_IS: weight exponent -h log2e z - h^2/2 log2e
     R den, inv_den, clip;  // TK_LR
     R fwd;        // TK_CV: E[S_T]
+    R hs, hk;     // FP32 pair units: 0.5 sg spot, -0.5 sg strike (sg = +1 call, -1 put)
     double n;     // TK_STRAT: strata (= paths)
     int32_t ot;
 };
@@ -931,14 +932,53 @@ __device__ __forceinline__ double term_unit(const TermC<R>& k, R z) {
     }
 }
 
+// The same units in FP32 for the Philox path's group sums: the pair mean and
+// the LR difference are formed in FP32 instead of FP64.  The pair mean is
+//   max(hs 2^(A + B z) + hk, 0) + max(hs 2^(A - B z) + hk, 0)
+// with hs = sg spot / 2 and hk = -sg strike / 2: the call/put sign and the
+// halving (exact in binary) sit in the constants, so each path costs one
+// exp2, one FMA and one max, with no per-path select.
+template <int KIND>
+__device__ __forceinline__ float term_unit_f32(const TermC<float>& k, float z) {
+    if constexpr (KIND == TK_PAIR) {
+        const float up = __builtin_fmaf(k.hs, q_exp2(__builtin_fmaf(k.B, z, k.A)), k.hk);
+        const float dn = __builtin_fmaf(k.hs, q_exp2(__builtin_fmaf(-k.B, z, k.A)), k.hk);
+        return (up > 0.0f ? up : 0.0f) + (dn > 0.0f ? dn : 0.0f);
+    } else if constexpr (KIND == TK_LR) {
+        const float e = k.B * z;
+        const float up = payoff(k.spot * q_exp2(k.A + e), k.strike, k.ot);
+        const float dn = payoff(k.spot * q_exp2(k.A - e), k.strike, k.ot);
+        return 0.5f * (up - dn) * lr_score(z, k.den, k.inv_den, k.clip);
+    } else if constexpr (KIND == TK_IS) {
+        const float st = k.spot * q_exp2(q_fma(k.B, z, k.Ah));
+        return payoff(st, k.strike, k.ot) * q_exp2(q_fma(k.hn, z, k.hc));
+    } else {
+        const float e = q_exp2(q_fma(k.B, z, k.A));
+        const float st = k.spot * e;
+        if (k.ot == QK_CALL) return st > k.strike ? e : 0.0f;
+        return st < k.strike ? -e : 0.0f;
+    }
+}
+
 // One tile's units: lane handles Philox groups g0 + lane + j*kBlock (4 units
 // each).  x1, x2, xy accumulate the control (TK_CV only).
+//
+// FP32 paths (TK_PAIR, TK_LR, TK_IS, TK_PW) sum each group's four centred
+// units and their squares in FP32 and add the two group sums to the lane's
+// FP64 sums: two FP64 additions per four units instead of five FP64
+// operations per unit.  The centring uses float(shift); the lane adds
+// count * (float(shift) - shift) at the end, so s1 stays the sum of
+// unit - shift.  The FP32 group sums round at 2^-24 of a few units, far below
+// any standard error the sums produce.
 template <typename R, int MODE, int KIND, class PS, class SS>
 __device__ __forceinline__ void term_tile(const TermC<R>& k, const PS& ph, const SS& ss,
                                           int64_t g0, int32_t lane_groups, int32_t units,
                                           double shift, double& s1, double& s2, double& x1,
                                           double& x2, double& xy) {
     constexpr bool from_u = KIND == TK_STRAT || KIND == TK_SOBOL || KIND == TK_HALTON;
+    constexpr bool f32_sums = sizeof(R) == 4 && !from_u && KIND != TK_CV;
+    const float sh = float(shift);
+    int32_t cnt = 0;
     for (int32_t j = 0; j < lane_groups; ++j) {
         const int64_t g = g0 + threadIdx.x + int64_t(j) * kBlock;
         if (g * 4 >= units) break;
@@ -955,6 +995,28 @@ __device__ __forceinline__ void term_tile(const TermC<R>& k, const PS& ph, const
         } else if constexpr (!from_u) {
             if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_block(uint32_t(g));
             else z = ss.term_block(uint32_t(g));
+        }
+        if constexpr (f32_sums) {
+            const int32_t left = int32_t(units - g * 4);  // >= 1 here
+            float t1 = 0.0f, t2 = 0.0f;
+            auto add = [&](int c) {
+                const float f = term_unit_f32<KIND>(k, z.z[c]) - sh;
+                t1 += f;
+                t2 = __builtin_fmaf(f, f, t2);
+            };
+            if (left >= 4) {  // every group but an option's last
+#pragma unroll
+                for (int c = 0; c < 4; ++c) add(c);
+                cnt += 4;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (c < left) add(c);
+                cnt += left;
+            }
+            s1 += double(t1);
+            s2 += double(t2);
+            continue;
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -983,6 +1045,7 @@ __device__ __forceinline__ void term_tile(const TermC<R>& k, const PS& ph, const
             }
         }
     }
+    if constexpr (f32_sums) s1 = __builtin_fma(double(cnt), double(sh) - shift, s1);
 }
 
 template <typename R, int MODE>
@@ -1019,6 +1082,9 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
         k.inv_den = R(1.0 / rec.aux[2]);
         k.clip = R(rec.aux[0]);
         k.fwd = R(rec.aux[0]);
+        const double sg = rec.ot == QK_CALL ? 0.5 : -0.5;
+        k.hs = R(sg * rec.spot);
+        k.hk = R(-sg * rec.strike);
         k.n = double(paths);
         const double shift = rec.shift;
         const bool anti = method_is_antithetic_terminal(method);
