@@ -138,17 +138,15 @@ double philox_path_normal(uint64_t seed, uint32_t p, uint32_t d) {
     return z[d & 3u];
 }
 
-// Heston (production) stream: step j of path p uses slot j%3 of block
-// (p, j/3, 0, 0).  u1 = (2m + 1) 2^-24 with m the low 23 bits of word j%3;
-// the angle fraction is the low 23 bits of w3 (slot 0), or 15 bits built from
-// bits 24-30 of w0 and 24-31 of w1 (slot 1) / of w2 and w3 (slot 2), over
-// 2^23.  (z1, z2) = sqrt(-2 ln u1) (cos 2 pi f, sin 2 pi f) in FP64.
-void philox_heston_normals(uint64_t seed, uint32_t p, uint32_t j, double* z1, double* z2) {
-    const uint32_t in[4] = {p, j / 3u, 0u, 0u};
+// Three-pair layout of one Philox block (counter in[], key = seed): slot s
+// takes u1 = (2m + 1) 2^-24 with m the low 23 bits of word s; the angle
+// fraction is the low 23 bits of w3 (slot 0), or 15 bits built from bits
+// 24-30 of w0 and 24-31 of w1 (slot 1) / of w2 and w3 (slot 2), over 2^23.
+// (z1, z2) = sqrt(-2 ln u1) (cos 2 pi f, sin 2 pi f) in FP64.
+void philox_slot_pair(const uint32_t in[4], uint64_t seed, uint32_t s, double* z1, double* z2) {
     const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
     uint32_t w[4];
     philox10(in, key, w);
-    const uint32_t s = j % 3u;
     const double u1 = (2.0 * double(w[s] & 0x007FFFFFu) + 1.0) * 0x1p-24;
     uint32_t fb;
     if (s == 0) fb = w[3] & 0x007FFFFFu;
@@ -161,6 +159,13 @@ void philox_heston_normals(uint64_t seed, uint32_t p, uint32_t j, double* z1, do
     *z2 = rad * std::sin(ang);
 }
 
+// Heston (production) stream: step j of path p uses slot j%3 of block
+// (p, j/3, 0, 0).
+void philox_heston_normals(uint64_t seed, uint32_t p, uint32_t j, double* z1, double* z2) {
+    const uint32_t in[4] = {p, j / 3u, 0u, 0u};
+    philox_slot_pair(in, seed, j % 3u, z1, z2);
+}
+
 // GBM / local-vol stepwise (production) stream: step j of path p takes
 // component j%2 of the Heston pair of step j/2, i.e. slot (j%6)/2 of block
 // (p, j/6, 0, 0), the cosine for even j and the sine for odd j: six normals
@@ -171,11 +176,14 @@ double philox_gbm_normal(uint64_t seed, uint32_t p, uint32_t j) {
     return (j & 1u) ? z2 : z1;
 }
 
-// Terminal estimators: draw k: block (k/4, 0), tag 1, component k%4.
+// Terminal estimators: draw k is component k%6 of block (k/6, 0, 0, 1) in
+// the three-pair layout: slot (k%6)/2, the cosine for even k, the sine for
+// odd k.
 double philox_terminal_normal(uint64_t seed, uint32_t k) {
-    double z[4];
-    philox_normals4(k >> 2, 0u, 1u, seed, z);
-    return z[k & 3u];
+    const uint32_t in[4] = {k / 6u, 0u, 0u, 1u};
+    double z1, z2;
+    philox_slot_pair(in, seed, (k % 6u) / 2u, &z1, &z2);
+    return (k & 1u) ? z2 : z1;
 }
 
 struct Moments {  // running sums of payoff units

@@ -95,7 +95,7 @@ struct Device {
     double ms_total = -1.0, ms_sim = -1.0;
 
     SimShape shape_step{8, 0};
-    SimShape shape_term{32, 0};
+    SimShape shape_term{24, 0};
 
     // instantiated launch graphs of the host-memory MC path, keyed by
     // everything their nodes depend on (sizes, shapes and buffer addresses)

@@ -81,10 +81,10 @@ void init_device() {
     const int per_cu = std::max(1, env_int("QK_GRID_PER_CU", 8));
     d->shape_step.lane_units = std::max(1, env_int("QK_LANE_PATHS", 8));
     d->shape_step.grid = d->cus * per_cu;
-    // 32 Philox groups (128 antithetic pairs) per lane: the autotuner's pick
-    // for 4096 x 100k-path options, where the per-tile fetch and lookup
+    // 24 groups (144 antithetic pairs) per lane: an autotuner pick for
+    // 4096 x 100k-path options, where the per-tile fetch and lookup
     // otherwise weigh on tiles of a few microseconds
-    d->shape_term.lane_units = std::max(1, env_int("QK_LANE_GROUPS", 32));
+    d->shape_term.lane_units = std::max(1, env_int("QK_LANE_GROUPS", 24));
     d->shape_term.grid = d->cus * per_cu;
     if (env_int("QK_AUTOTUNE", 0)) apply_cached_shapes(d);
     g_dev = d;
@@ -128,7 +128,7 @@ int32_t launch_pipeline(Device* d, const BatchIn& in, int32_t n, const int32_t* 
     Partial* parts = static_cast<Partial*>(d->partials.p);
     const SimShape& shp = family == FAM_TERMINAL ? d->shape_term : d->shape_step;
     if (timed) QK_HIP(hipEventRecord(d->ev[0], d->stream));
-    QK_HIP(launch_prepare(d->stream, in, n_dev, stream_off, tile_units_terminal(d->shape_term),
+    QK_HIP(launch_prepare(d->stream, in, n_dev, stream_off, tile_groups_terminal(d->shape_term),
                           tile_units_stepwise(d->shape_step), recs, ntiles));
     QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p, capacity));
     if (timed) QK_HIP(hipEventRecord(d->ev[1], d->stream));
@@ -162,7 +162,7 @@ void record_timing_total(Device* d) {
 // fewer tiles than this).
 int64_t tile_bound(const HostBatch& hb, int32_t n, const Device* d) {
     const int64_t tu_s = tile_units_stepwise(d->shape_step);
-    const int64_t tu_t = tile_units_terminal(d->shape_term);
+    const int64_t tu_t = int64_t(tile_groups_terminal(d->shape_term)) * 4;  // smallest group
     int64_t total = 0;
     for (int32_t i = 0; i < n; ++i) {
         const int64_t p = std::max<int32_t>(hb.paths[i], 0);
@@ -531,7 +531,7 @@ int32_t mc_device_locked(Device* d, const BatchIn& in, int32_t n, const int32_t*
     int32_t* tstart = static_cast<int32_t*>(d->tile_start.p);
     int32_t* ctr = static_cast<int32_t*>(d->counters.p);
     QK_HIP(hipEventRecord(d->ev[0], d->stream));
-    QK_HIP(launch_prepare(d->stream, in, n_dev, nullptr, tile_units_terminal(d->shape_term),
+    QK_HIP(launch_prepare(d->stream, in, n_dev, nullptr, tile_groups_terminal(d->shape_term),
                           tile_units_stepwise(d->shape_step), recs, ntiles));
     QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p, 0x7fffffff));
     QK_HIP(hipMemcpyAsync(d->pin_out.p, ctr, 16, hipMemcpyDeviceToHost, d->stream));

@@ -41,13 +41,14 @@ struct SimShape {
 constexpr int kBlock = 256;  // threads per workgroup (4 wave64)
 
 inline int32_t tile_units_stepwise(const SimShape& s) { return kBlock * s.lane_units; }
-inline int32_t tile_units_terminal(const SimShape& s) { return kBlock * s.lane_units * 4; }
+// terminal tiles hold kBlock * lane_units groups of term_group_units(method) units
+inline int32_t tile_groups_terminal(const SimShape& s) { return kBlock * s.lane_units; }
 
 // counters[0] = next tile, counters[1] = tiles to run, counters[2] = overflow
 // (1: more tiles than `capacity`, 2: more than 2^31-1; no tile runs and the
 // finalizer writes NaN), counters[3] = tiles needed (clamped to 2^31-1)
 hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev,
-                          const int64_t* stream_off, int32_t tu_terminal, int32_t tu_stepwise,
+                          const int64_t* stream_off, int32_t tg_terminal, int32_t tu_stepwise,
                           OptionRec* recs, int32_t* ntiles);
 // scratch: (n + 1)/1024 + 1 int64 block sums
 hipError_t launch_scan(hipStream_t st, int32_t n, const int32_t* n_dev, const int32_t* ntiles,

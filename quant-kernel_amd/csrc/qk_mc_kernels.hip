@@ -67,6 +67,10 @@ template <typename R>
 struct Quad {
     R z[4];
 };
+template <typename R, int N>
+struct Draws {
+    R z[N];
+};
 
 // Production: Philox blocks keyed by the option seed (wave-uniform key).
 struct PhiloxSrc {
@@ -84,13 +88,15 @@ struct PhiloxSrc {
         u[2] = (double(w.z) + 0.5) * 0x1p-32;
         u[3] = (double(w.w) + 0.5) * 0x1p-32;
     }
-    __device__ __forceinline__ Quad<float> term_block(uint32_t g) const {
-        const Normal4 n = philox_normals(g, 0u, 1u, k0, k1);
-        return Quad<float>{{n.z0, n.z1, n.z2, n.z3}};
+    // normals k = 6b .. 6b+5 of a terminal estimator: the three pairs of
+    // block (b, 0, 0, 1)
+    __device__ __forceinline__ Draws<float, 6> term_normals6(uint32_t b) const {
+        const Normal6 n = philox_normals6(philox4x32_10(b, 0u, 1u, k0, k1));
+        return Draws<float, 6>{{n.z[0], n.z[1], n.z[2], n.z[3], n.z[4], n.z[5]}};
     }
     __device__ __forceinline__ float term_draw(uint32_t k) const {
-        const Quad<float> q = term_block(k >> 2);
-        return q.z[k & 3u];
+        const Draws<float, 6> q = term_normals6(k / 6u);
+        return q.z[k % 6u];
     }
 };
 
@@ -110,11 +116,13 @@ struct StreamSrc {
         }
         return q;
     }
-    __device__ __forceinline__ Quad<R> term_block(uint32_t g) const {
-        Quad<R> q;
+    // draws k = N g .. N g + N-1
+    template <int N>
+    __device__ __forceinline__ Draws<R, N> term_draws(uint32_t g) const {
+        Draws<R, N> q;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int64_t k = int64_t(g) * 4 + c;
+        for (int c = 0; c < N; ++c) {
+            const int64_t k = int64_t(g) * N + c;
             q.z[c] = k < ndraws ? base[k] : R(0);
         }
         return q;
@@ -137,7 +145,7 @@ __device__ __forceinline__ int32_t model_to_method(int32_t m) {
 
 __global__ __launch_bounds__(256) void k_prepare(BatchIn in, const int32_t* __restrict__ n_dev,
                                                  const int64_t* __restrict__ stream_off,
-                                                 int32_t tu_terminal, int32_t tu_stepwise,
+                                                 int32_t tg_terminal, int32_t tu_stepwise,
                                                  OptionRec* __restrict__ recs,
                                                  int32_t* __restrict__ ntiles) {
     const int32_t n = *n_dev;
@@ -169,7 +177,8 @@ __global__ __launch_bounds__(256) void k_prepare(BatchIn in, const int32_t* __re
             rec.closed = qnan_d();
             rec.ntiles = 0;
         } else {
-            prepare_option(o, method_is_terminal(m) ? tu_terminal : tu_stepwise, rec);
+            prepare_option(o, method_is_terminal(m) ? tg_terminal * term_group_units(m) : tu_stepwise,
+                           rec);
         }
         if (rec.family == FAM_NONE) rec.ntiles = 0;
         if (stream_off) rec.stream_off = stream_off[i];
@@ -512,17 +521,6 @@ struct HestonPx {
 // needs no canonicalisation of the loop-invariant floor inside the loop.
 __device__ __forceinline__ float floor_max(float x, float f) {
     return __builtin_amdgcn_fmed3f(x, f, 0x1.fffffep+127f);
-}
-
-__device__ __forceinline__ float u23(uint32_t w) {
-    return __builtin_bit_cast(float, (w & 0x007FFFFFu) | 0x3F800000u) - 0x1.fffffep-1f;
-}
-__device__ __forceinline__ float rev23(uint32_t bits) {  // 1 + fraction, in [1, 2)
-    return __builtin_bit_cast(float, (bits & 0x007FFFFFu) | 0x3F800000u);
-}
-// bytes {0, w_hi.byte3, w_lo.byte3, 0} (v_perm_b32)
-__device__ __forceinline__ uint32_t top_bytes(uint32_t w_hi, uint32_t w_lo) {
-    return __builtin_amdgcn_perm(w_hi, w_lo, 0x0C07030Cu);
 }
 
 // sqrt(v) r' = sqrt(-v log2 u1): one square root per step instead of two.
@@ -960,8 +958,10 @@ __device__ __forceinline__ float term_unit_f32(const TermC<float>& k, float z) {
     }
 }
 
-// One tile's units: lane handles Philox groups g0 + lane + j*kBlock (4 units
-// each).  x1, x2, xy accumulate the control (TK_CV only).
+// One tile's units: lane handles groups g0 + lane + j*kBlock of GU units:
+// six (one Philox block of three Box-Muller pairs) for the estimators that
+// draw normals, four for the uniform-driven ones.  x1, x2, xy accumulate the
+// control (TK_CV only).
 //
 // FP32 paths (TK_PAIR, TK_LR, TK_IS, TK_PW) sum each group's four centred
 // units and their squares in FP32 and add the two group sums to the lane's
@@ -976,41 +976,42 @@ __device__ __forceinline__ void term_tile(const TermC<R>& k, const PS& ph, const
                                           double shift, double& s1, double& s2, double& x1,
                                           double& x2, double& xy) {
     constexpr bool from_u = KIND == TK_STRAT || KIND == TK_SOBOL || KIND == TK_HALTON;
+    constexpr int GU = from_u ? 4 : 6;  // units per group (term_group_units)
     constexpr bool f32_sums = sizeof(R) == 4 && !from_u && KIND != TK_CV;
     const float sh = float(shift);
     int32_t cnt = 0;
     for (int32_t j = 0; j < lane_groups; ++j) {
         const int64_t g = g0 + threadIdx.x + int64_t(j) * kBlock;
-        if (g * 4 >= units) break;
-        Quad<R> z;
+        if (g * GU >= units) break;
+        Draws<R, GU> z;
         double uu[4] = {0.0, 0.0, 0.0, 0.0};
         if constexpr (KIND == TK_STRAT) {
             if constexpr (MODE == SIM_PHILOX_F32) {
                 ph.term_uniforms(uint32_t(g), uu);
             } else {
-                const Quad<R> q = ss.term_block(uint32_t(g));  // the stream holds uniforms
+                const Draws<R, 4> q = ss.template term_draws<4>(uint32_t(g));  // uniforms
 #pragma unroll
                 for (int c = 0; c < 4; ++c) uu[c] = double(q.z[c]);
             }
         } else if constexpr (!from_u) {
-            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_block(uint32_t(g));
-            else z = ss.term_block(uint32_t(g));
+            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_normals6(uint32_t(g));
+            else z = ss.template term_draws<6>(uint32_t(g));
         }
         if constexpr (f32_sums) {
-            const int32_t left = int32_t(units - g * 4);  // >= 1 here
+            const int32_t left = int32_t(units - g * GU);  // >= 1 here
             float t1 = 0.0f, t2 = 0.0f;
             auto add = [&](int c) {
                 const float f = term_unit_f32<KIND>(k, z.z[c]) - sh;
                 t1 += f;
                 t2 = __builtin_fmaf(f, f, t2);
             };
-            if (left >= 4) {  // every group but an option's last
+            if (left >= GU) {  // every group but an option's last
 #pragma unroll
-                for (int c = 0; c < 4; ++c) add(c);
-                cnt += 4;
+                for (int c = 0; c < GU; ++c) add(c);
+                cnt += GU;
             } else {
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
+                for (int c = 0; c < GU - 1; ++c)
                     if (c < left) add(c);
                 cnt += left;
             }
@@ -1019,8 +1020,8 @@ __device__ __forceinline__ void term_tile(const TermC<R>& k, const PS& ph, const
             continue;
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int64_t unit = g * 4 + c;
+        for (int c = 0; c < GU; ++c) {
+            const int64_t unit = g * GU + c;
             if (unit < units) {
                 double f;
                 if constexpr (from_u) {
@@ -1067,7 +1068,7 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
         const OptionRec& rec = recs[opt];
         const int32_t units = rec.units, paths = rec.paths, method = rec.method;
         const int32_t local_tile = tile - rec.tile_first;
-        const int64_t g0 = int64_t(local_tile) * kBlock * lane_groups;  // first Philox group
+        const int64_t g0 = int64_t(local_tile) * kBlock * lane_groups;  // first group
         TermC<R> k;
         k.spot = R(rec.spot);
         k.strike = R(rec.strike);
@@ -1259,10 +1260,10 @@ hipError_t launch_philox_probe(hipStream_t st, uint64_t seed, uint32_t c0, uint3
 // host launchers
 // ----------------------------------------------------------------------------
 hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev,
-                          const int64_t* stream_off, int32_t tu_terminal, int32_t tu_stepwise,
+                          const int64_t* stream_off, int32_t tg_terminal, int32_t tu_stepwise,
                           OptionRec* recs, int32_t* ntiles) {
     hipLaunchKernelGGL(k_prepare, dim3(1024), dim3(256), 0, st, in, n_dev, stream_off,
-                       tu_terminal, tu_stepwise, recs, ntiles);
+                       tg_terminal, tu_stepwise, recs, ntiles);
     return hipGetLastError();
 }
 

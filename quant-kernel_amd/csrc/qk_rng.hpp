@@ -2,15 +2,26 @@
 //
 // Production stream: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed
 // by the option's 64-bit seed, followed by Box-Muller (1958) in FP32 on the
-// hardware transcendental units.  Draw d of path p of an LSMC or MLMC path
-// is component d%4 of block (p, d/4, 0, level); draw k of a terminal
-// (antithetic) estimator is component k%4 of block (k/4, 0, 0, 1).  Words
-// (w0,w1) give components 0,1 and (w2,w3) components 2,3 of one block:
+// hardware transcendental units.
+//
+// Four normals per block: draw d of path p of an LSMC or MLMC path is
+// component d%4 of block (p, d/4, 0, level).  Words (w0,w1) give components
+// 0,1 and (w2,w3) components 2,3:
 //   u1 = fma(float(w_even), 2^-32, 2^-33)           (FP32, in (0, 1])
 //   f  = (w_odd & 0x7FFFFF) / 2^23                   (exact, in [0, 1))
 //   (z_even, z_odd) = sqrt(-2 ln u1) * (cos 2 pi f, sin 2 pi f)
-// Heston and stepwise GBM paths use three pairs per block instead (u1 from 23
-// bits, angles from the remaining bits), built in the path kernels.
+//
+// Six normals per block (philox_normals6): three pairs, slot s taking
+//   u1 = (2m + 1) 2^-24, m = the low 23 bits of w_s  (exact in FP32)
+// and its angle from bits that no slot's m uses:
+//   slot 0: the low 23 bits of w3;
+//   slot 1: bits 24-30 of w0 and 24-31 of w1 (15 bits);
+//   slot 2: bits 24-30 of w2 and 24-31 of w3 (15 bits).
+// The radius is capped at sqrt(48 ln 2) = 5.77 sigma.  Draw k of a terminal
+// estimator is component k%6 of block (k/6, 0, 0, 1) (cosine of slot
+// (k%6)/2 for even k, sine for odd k).  Heston paths take step j from slot
+// j%3 of block (p, j/3, 0, 0) (both components), stepwise GBM paths step j
+// from component j%2 of the pair of Heston step j/2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -89,6 +100,38 @@ __device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float& z0
 struct Normal4 {
     float z0, z1, z2, z3;
 };
+
+// Three-pair layout helpers.  u23: (2m + 1) 2^-24 from the low 23 bits m of
+// w, exact in FP32 and in (0, 1).  rev23: 1 + fraction in [1, 2) (an angle in
+// revolutions for v_sin/v_cos).  top_bytes: {0, w_hi.byte3, w_lo.byte3, 0}
+// (one v_perm_b32), the 15-bit angle fraction of slots 1 and 2.
+__device__ __forceinline__ float u23(uint32_t w) {
+    return __builtin_bit_cast(float, (w & 0x007FFFFFu) | 0x3F800000u) - 0x1.fffffep-1f;
+}
+__device__ __forceinline__ float rev23(uint32_t bits) {
+    return __builtin_bit_cast(float, (bits & 0x007FFFFFu) | 0x3F800000u);
+}
+__device__ __forceinline__ uint32_t top_bytes(uint32_t w_hi, uint32_t w_lo) {
+    return __builtin_amdgcn_perm(w_hi, w_lo, 0x0C07030Cu);
+}
+
+struct Normal6 {
+    float z[6];
+};
+
+__device__ __forceinline__ void box_muller23(float u1, float rev, float& z0, float& z1) {
+    const float rad = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u1) * -1.3862943611198906f);
+    z0 = rad * __builtin_amdgcn_cosf(rev);
+    z1 = rad * __builtin_amdgcn_sinf(rev);
+}
+
+__device__ __forceinline__ Normal6 philox_normals6(const U4& w) {
+    Normal6 n;
+    box_muller23(u23(w.x), rev23(w.w), n.z[0], n.z[1]);
+    box_muller23(u23(w.y), rev23(top_bytes(w.x, w.y)), n.z[2], n.z[3]);
+    box_muller23(u23(w.z), rev23(top_bytes(w.z, w.w)), n.z[4], n.z[5]);
+    return n;
+}
 
 __device__ __forceinline__ Normal4 philox_normals(uint32_t c0, uint32_t c1, uint32_t c3,
                                                   uint32_t k0, uint32_t k1) {

@@ -195,6 +195,50 @@ def test_gbm_stream_is_heston_pairs(orc):
     assert orc.lib.orc_philox_gbm_normal(seed, 5, 7) == pytest.approx(rad * math.sin(ang), rel=1e-13, abs=1e-13)
 
 
+def test_terminal_stream_exact_mapping(orc):
+    """Draw k of a terminal estimator: component k%6 of Philox block
+    (k/6, 0, 0, 1) in the three-pair layout, i.e. the cosine (even k) or sine
+    (odd k) of slot (k%6)/2.  Restated from the words with numpy."""
+    seed = 0x5EED_0F_7E57_C0DE
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    for k in list(range(18)) + [6 * 12345 + 5, 6 * (2**31 + 3) + 2]:
+        k &= 0xFFFFFFFF
+        w = orc.philox([k // 6, 0, 0, 1], key)
+        s = (k % 6) // 2
+        u1 = (2.0 * (w[s] & 0x7FFFFF) + 1.0) / 2.0**24
+        if s == 0:
+            fb = w[3] & 0x7FFFFF
+        else:
+            hi, lo = (w[0], w[1]) if s == 1 else (w[2], w[3])
+            fb = ((hi >> 24) & 0x7F) << 16 | (lo >> 24) << 8
+        rad = math.sqrt(-2.0 * math.log(u1))
+        ang = 2 * math.pi * fb / 2.0**23
+        want = rad * (math.sin(ang) if k % 2 else math.cos(ang))
+        assert orc.lib.orc_philox_terminal_normal(seed, k) == pytest.approx(want, rel=1e-13, abs=1e-13), k
+    # the terminal stream (fourth counter word 1) is not the Heston stream of
+    # the same block (fourth word 0)
+    assert orc.lib.orc_philox_terminal_normal(seed, 0) != orc.heston_normals(seed, 0, 0)[0]
+
+
+def test_terminal_stream_distribution(orc):
+    n = 48000
+    z = np.array([orc.lib.orc_philox_terminal_normal(2024, k) for k in range(n)])
+    assert abs(z.mean()) < 5 / math.sqrt(n)
+    assert abs(z.var() - 1) < 5 * math.sqrt(2 / n)
+    kurt = ((z - z.mean()) ** 4).mean() / z.var() ** 2
+    assert abs(kurt - 3) < 0.15
+    zs = np.sort(z)
+    cdf = 0.5 * np.array([math.erfc(-x / math.sqrt(2)) for x in zs])
+    assert np.max(np.abs(cdf - np.arange(1, n + 1) / n)) < 1.63 / math.sqrt(n) * 1.5
+    # the six components of a block are uncorrelated, in value and in square
+    b = z.reshape(-1, 6)
+    m = b.shape[0]
+    for a, c in ((0, 1), (0, 2), (1, 3), (2, 4), (4, 5), (0, 5)):
+        assert abs(np.corrcoef(b[:, a], b[:, c])[0, 1]) < 5 / math.sqrt(m)
+        assert abs(np.corrcoef(b[:, a] ** 2, b[:, c] ** 2)[0, 1]) < 5 / math.sqrt(m)
+    assert np.abs(z).max() <= math.sqrt(48 * math.log(2)) + 1e-12
+
+
 def test_gbm_stream_distribution(orc):
     n_paths, n_steps = 4000, 12
     z = np.array([[orc.lib.orc_philox_gbm_normal(4242, p, j) for j in range(n_steps)] for p in range(n_paths)])
