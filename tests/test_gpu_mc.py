@@ -456,9 +456,10 @@ def test_device_entry_points_equal_host_paths_and_replay(qk):
 
 def test_device_partials_grow_beyond_capacity(qk):
     """A device batch needing more tiles than the partials buffer holds runs
-    again after growing it: 40 options x (2^31 - 1) paths = 5.2M tiles."""
+    again after growing it: 80 options x (2^31 - 1) paths = 2.3M tiles at the
+    default shape (24 six-pair groups per lane), against 2^20 at first."""
     from hip_mem import DevArray
-    n = 40
+    n = 80
     d = wl.config0(n, paths=1)
     d["paths"] = np.full(n, 2**31 - 1, np.int32)
     t = _to_dev(d, GBM_KEYS)

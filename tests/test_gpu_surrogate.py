@@ -125,8 +125,9 @@ def test_pricing_sources_and_fallback(qk):
         + max(0.0, d["strike"][9] - d["spot"][9]) * (d["option_type"][9] == 1)
     for i in (5, 6, 7, 10):
         assert price[i] == mc[i]  # the same MC pipeline, bitwise
-    feats = np.stack([np.log(d["strike"] / d["spot"]), d["t"], d["r"], d["kappa"], d["theta"], d["sigma"], d["rho"],
-                      d["v0"]], axis=1)
+    with np.errstate(invalid="ignore"):  # query 8 has a negative spot (invalid, src 3)
+        logm = np.log(d["strike"] / d["spot"])
+    feats = np.stack([logm, d["t"], d["r"], d["kappa"], d["theta"], d["sigma"], d["rho"], d["v0"]], axis=1)
     net = src == 0
     assert net.sum() >= 50
     y = qk.sur_predict(feats[net])
