@@ -118,16 +118,28 @@ __device__ __forceinline__ int lsmc_tiles(int32_t paths) {
 
 }  // namespace
 
-// 16 bytes of R (4 FP32 or 2 FP64 values) loaded or stored by one instruction
+// 16 bytes of R (4 FP32 or 2 FP64 values) loaded or stored by one
+// instruction, marked non-temporal: a sweep touches each of S and the
+// cashflows once, and no cache holds them until the next sweep (2.7% faster
+// than cached accesses for 1024 options x 100k paths x 50 dates)
 template <typename R>
 struct alignas(16) LsmcVec {
     static constexpr int N = int(16 / sizeof(R));
+    typedef R vt __attribute__((ext_vector_type(N)));
     R v[N] = {};
     __device__ __forceinline__ static LsmcVec load(const R* p) {
-        return *reinterpret_cast<const LsmcVec*>(__builtin_assume_aligned(p, 16));
+        const vt x =
+            __builtin_nontemporal_load(reinterpret_cast<const vt*>(__builtin_assume_aligned(p, 16)));
+        LsmcVec o;
+#pragma unroll
+        for (int i = 0; i < N; ++i) o.v[i] = x[i];
+        return o;
     }
     __device__ __forceinline__ void store(R* p) const {
-        *reinterpret_cast<LsmcVec*>(__builtin_assume_aligned(p, 16)) = *this;
+        vt x;
+#pragma unroll
+        for (int i = 0; i < N; ++i) x[i] = v[i];
+        __builtin_nontemporal_store(x, reinterpret_cast<vt*>(__builtin_assume_aligned(p, 16)));
     }
 };
 // 16-byte groups whose loads are issued before the first is used (two was
