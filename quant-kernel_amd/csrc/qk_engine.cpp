@@ -115,26 +115,41 @@ bool is_terminal_method(int32_t m) { return method_is_terminal(m); }
 
 size_t align64(size_t x) { return (x + 63) & ~size_t(63); }
 
+// Device address of byte `offset` of a pinned host allocation (the pointer
+// hipHostMalloc returned), or null.
+int32_t* host_flags_ptr(void* pinned, size_t offset) {
+    void* dp = nullptr;
+    if (hipHostGetDevicePointer(&dp, pinned, 0) != hipSuccess || !dp) return nullptr;
+    return reinterpret_cast<int32_t*>(static_cast<char*>(dp) + offset);
+}
+
 // Launch prepare -> scan -> simulate -> finalize on d's stream.  The caller
-// holds d->mu.  `family` selects the simulation kernel.
+// holds d->mu.  `family` selects the simulation kernel.  With n_dev null the
+// batch size is n, which prepare stores in counters[8] for the later kernels.
+// host_flags (device address of pinned host memory, or null) receives the
+// scan's counters[0..3] from the finalizer.
 int32_t launch_pipeline(Device* d, const BatchIn& in, int32_t n, const int32_t* n_dev,
                         const int64_t* stream_off, int32_t family, int32_t mode,
                         const void* stream, double* out_price, double* out_se,
-                        std::string* err, bool timed = true, int32_t capacity = 0x7fffffff) {
+                        std::string* err, bool timed = true, int32_t capacity = 0x7fffffff,
+                        int32_t* host_flags = nullptr) {
     OptionRec* recs = static_cast<OptionRec*>(d->recs.p);
     int32_t* ntiles = static_cast<int32_t*>(d->ntiles.p);
     int32_t* tstart = static_cast<int32_t*>(d->tile_start.p);
     int32_t* ctr = static_cast<int32_t*>(d->counters.p);
     Partial* parts = static_cast<Partial*>(d->partials.p);
     const SimShape& shp = family == FAM_TERMINAL ? d->shape_term : d->shape_step;
+    int32_t* n_slot = n_dev ? nullptr : ctr + 8;
+    if (!n_dev) n_dev = n_slot;
     if (timed) QK_HIP(hipEventRecord(d->ev[0], d->stream));
-    QK_HIP(launch_prepare(d->stream, in, n_dev, stream_off, tile_groups_terminal(d->shape_term),
-                          tile_units_stepwise(d->shape_step), recs, ntiles));
+    QK_HIP(launch_prepare(d->stream, in, n_slot ? nullptr : n_dev, n, n_slot, stream_off,
+                          tile_groups_terminal(d->shape_term), tile_units_stepwise(d->shape_step),
+                          recs, ntiles));
     QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p, capacity));
     if (timed) QK_HIP(hipEventRecord(d->ev[1], d->stream));
     QK_HIP(launch_simulate(d->stream, family, mode, shp, recs, tstart, n_dev, ctr, parts, stream));
     if (timed) QK_HIP(hipEventRecord(d->ev[2], d->stream));
-    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, ctr, out_price, out_se));
+    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, ctr, out_price, out_se, host_flags));
     if (timed) QK_HIP(hipEventRecord(d->ev[3], d->stream));
     return QK_OK;
 }
@@ -470,17 +485,20 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
     }
 
     const bool use_graph = graphed && graphs_active(d);
+    int32_t* hflags = host_flags_ptr(d->pin_out.p, ns * 16);
+    if (!hflags) {
+        if (err) *err = "pinned host memory has no device address";
+        return QK_ERR_RUNTIME_INIT;
+    }
     auto enqueue = [&]() -> int32_t {
         if (graphed)
             QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, in_copy, hipMemcpyHostToDevice, d->stream));
-        // inside a graph the pipeline records no events (they are taken around it)
+        // inside a graph the pipeline records no events (they are taken around it);
+        // the finalizer stores the scan's counters after the outputs in pin_out
         int32_t rc = launch_pipeline(d, bi, n, n_dev, d_off, family, sim_mode, d_stream, dprice, dse,
-                                     err, !use_graph);
+                                     err, !use_graph, 0x7fffffff, hflags);
         if (rc != QK_OK) return rc;
-        // outputs + the scan's overflow flag
         QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, ns * 16, hipMemcpyDeviceToHost, d->stream));
-        QK_HIP(hipMemcpyAsync(static_cast<char*>(d->pin_out.p) + ns * 16, d->counters.p, 16,
-                              hipMemcpyDeviceToHost, d->stream));
         return QK_OK;
     };
     int32_t rc;
@@ -531,8 +549,9 @@ int32_t mc_device_locked(Device* d, const BatchIn& in, int32_t n, const int32_t*
     int32_t* tstart = static_cast<int32_t*>(d->tile_start.p);
     int32_t* ctr = static_cast<int32_t*>(d->counters.p);
     QK_HIP(hipEventRecord(d->ev[0], d->stream));
-    QK_HIP(launch_prepare(d->stream, in, n_dev, nullptr, tile_groups_terminal(d->shape_term),
-                          tile_units_stepwise(d->shape_step), recs, ntiles));
+    QK_HIP(launch_prepare(d->stream, in, n_dev, 0, nullptr, nullptr,
+                          tile_groups_terminal(d->shape_term), tile_units_stepwise(d->shape_step),
+                          recs, ntiles));
     QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p, 0x7fffffff));
     QK_HIP(hipMemcpyAsync(d->pin_out.p, ctr, 16, hipMemcpyDeviceToHost, d->stream));
     QK_HIP(hipStreamSynchronize(d->stream));
@@ -550,7 +569,7 @@ int32_t mc_device_locked(Device* d, const BatchIn& in, int32_t n, const int32_t*
     QK_HIP(launch_simulate(d->stream, family, SIM_PHILOX_F32, shp, recs, tstart, n_dev, ctr, parts,
                            nullptr));
     QK_HIP(hipEventRecord(d->ev[2], d->stream));
-    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, ctr, out_price_dev, out_se_dev));
+    QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, ctr, out_price_dev, out_se_dev, nullptr));
     QK_HIP(hipEventRecord(d->ev[3], d->stream));
     return QK_OK;
 }
@@ -588,20 +607,20 @@ int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
     QK_HIP(d->partials.ensure(size_t(min_tiles) * sizeof(Partial)));
     const int32_t family =
         (!in.method && is_terminal_method(in.uniform_method)) ? FAM_TERMINAL : FAM_STEPWISE;
-    const int32_t* n_dev = static_cast<const int32_t*>(d->in.p);
+    int32_t* hflags = host_flags_ptr(d->pin_out.p, 0);
+    if (!hflags) {
+        if (err) *err = "pinned host memory has no device address";
+        return QK_ERR_RUNTIME_INIT;
+    }
     for (int attempt = 0; attempt < 2; ++attempt) {
         const int32_t cap =
             int32_t(std::min<size_t>(d->partials.cap / sizeof(Partial), size_t(0x7fffffff)));
-        std::memcpy(d->pin_in.p, &n, 4);
         const bool use_graph = attempt == 0 && graphs_active(d);
+        // n travels as a kernel argument; the finalizer stores the scan's
+        // counters in pin_out: the graph holds only the four kernels
         auto enqueue = [&]() -> int32_t {
-            QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, 4, hipMemcpyHostToDevice, d->stream));
-            int32_t r = launch_pipeline(d, in, n, n_dev, nullptr, family, SIM_PHILOX_F32, nullptr,
-                                        out_price_dev, out_se_dev, err, !use_graph, cap);
-            if (r != QK_OK) return r;
-            QK_HIP(hipMemcpyAsync(d->pin_out.p, d->counters.p, 16, hipMemcpyDeviceToHost,
-                                  d->stream));
-            return QK_OK;
+            return launch_pipeline(d, in, n, nullptr, nullptr, family, SIM_PHILOX_F32, nullptr,
+                                   out_price_dev, out_se_dev, err, !use_graph, cap, hflags);
         };
         if (use_graph) {
             const SimShape& st = d->shape_step;

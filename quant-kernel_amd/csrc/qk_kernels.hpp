@@ -47,9 +47,10 @@ inline int32_t tile_groups_terminal(const SimShape& s) { return kBlock * s.lane_
 // counters[0] = next tile, counters[1] = tiles to run, counters[2] = overflow
 // (1: more tiles than `capacity`, 2: more than 2^31-1; no tile runs and the
 // finalizer writes NaN), counters[3] = tiles needed (clamped to 2^31-1)
-hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev,
-                          const int64_t* stream_off, int32_t tg_terminal, int32_t tu_stepwise,
-                          OptionRec* recs, int32_t* ntiles);
+// n: *n_dev when n_dev is set, else n_host (then stored to n_out if set)
+hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev, int32_t n_host,
+                          int32_t* n_out, const int64_t* stream_off, int32_t tg_terminal,
+                          int32_t tu_stepwise, OptionRec* recs, int32_t* ntiles);
 // scratch: (n + 1)/1024 + 1 int64 block sums
 hipError_t launch_scan(hipStream_t st, int32_t n, const int32_t* n_dev, const int32_t* ntiles,
                        OptionRec* recs, int32_t* tile_start, int32_t* counters, void* scratch,
@@ -58,9 +59,11 @@ hipError_t launch_simulate(hipStream_t st, int32_t family, int32_t mode, const S
                            const OptionRec* recs, const int32_t* tile_start,
                            const int32_t* n_dev, int32_t* counters, Partial* partials,
                            const void* stream);
+// host_flags (nullable): device address of pinned host memory that receives
+// counters[0..3]
 hipError_t launch_finalize(hipStream_t st, const int32_t* n_dev, const OptionRec* recs,
                            const Partial* partials, const int32_t* counters, double* out_price,
-                           double* out_se);
+                           double* out_se, int32_t* host_flags);
 
 hipError_t launch_philox_probe(hipStream_t st, uint64_t seed, uint32_t c0, uint32_t c1,
                                uint32_t c3, int32_t count, uint32_t* words, float* normals);

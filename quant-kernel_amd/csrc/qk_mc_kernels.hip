@@ -143,12 +143,17 @@ __device__ __forceinline__ int32_t model_to_method(int32_t m) {
     }
 }
 
+// The batch size comes from n_dev when it is set (a count produced on the
+// device), else from n_host; with n_out set, prepare stores it there for the
+// later kernels of the batch, so a host-sized batch needs no copy of it.
 __global__ __launch_bounds__(256) void k_prepare(BatchIn in, const int32_t* __restrict__ n_dev,
+                                                 int32_t n_host, int32_t* __restrict__ n_out,
                                                  const int64_t* __restrict__ stream_off,
                                                  int32_t tg_terminal, int32_t tu_stepwise,
                                                  OptionRec* __restrict__ recs,
                                                  int32_t* __restrict__ ntiles) {
-    const int32_t n = *n_dev;
+    const int32_t n = n_dev ? *n_dev : n_host;
+    if (n_out && blockIdx.x == 0 && threadIdx.x == 0) *n_out = n;
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         RawOption o;
         int32_t m = in.method ? in.method[i] : in.uniform_method;
@@ -1136,14 +1141,19 @@ struct Neumaier {  // compensated FP64 sum (Neumaier 1974)
     __device__ __forceinline__ double sum() const { return s + c; }
 };
 
+// host_flags, when set, is pinned host memory (its device address): the
+// scan's counters[0..3] are stored there, so the host reads the overflow flag
+// after the stream completes without a copy of its own.
 __global__ __launch_bounds__(256) void k_finalize(const int32_t* __restrict__ n_dev,
                                                   const OptionRec* __restrict__ recs,
                                                   const Partial* __restrict__ partials,
                                                   const int32_t* __restrict__ counters,
                                                   double* __restrict__ out_price,
-                                                  double* __restrict__ out_se) {
+                                                  double* __restrict__ out_se,
+                                                  int32_t* __restrict__ host_flags) {
     const int32_t n = *n_dev;
     const bool overflow = counters[2] != 0;  // no tile ran: partials were not written
+    if (host_flags && blockIdx.x == 0 && threadIdx.x < 4) host_flags[threadIdx.x] = counters[threadIdx.x];
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const OptionRec& rec = recs[i];
         if (rec.family == FAM_NONE) {
@@ -1259,11 +1269,11 @@ hipError_t launch_philox_probe(hipStream_t st, uint64_t seed, uint32_t c0, uint3
 // ----------------------------------------------------------------------------
 // host launchers
 // ----------------------------------------------------------------------------
-hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev,
-                          const int64_t* stream_off, int32_t tg_terminal, int32_t tu_stepwise,
-                          OptionRec* recs, int32_t* ntiles) {
-    hipLaunchKernelGGL(k_prepare, dim3(1024), dim3(256), 0, st, in, n_dev, stream_off,
-                       tg_terminal, tu_stepwise, recs, ntiles);
+hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev, int32_t n_host,
+                          int32_t* n_out, const int64_t* stream_off, int32_t tg_terminal,
+                          int32_t tu_stepwise, OptionRec* recs, int32_t* ntiles) {
+    hipLaunchKernelGGL(k_prepare, dim3(1024), dim3(256), 0, st, in, n_dev, n_host, n_out,
+                       stream_off, tg_terminal, tu_stepwise, recs, ntiles);
     return hipGetLastError();
 }
 
@@ -1332,9 +1342,9 @@ hipError_t launch_simulate(hipStream_t st, int32_t family, int32_t mode, const S
 
 hipError_t launch_finalize(hipStream_t st, const int32_t* n_dev, const OptionRec* recs,
                            const Partial* partials, const int32_t* counters, double* out_price,
-                           double* out_se) {
+                           double* out_se, int32_t* host_flags) {
     hipLaunchKernelGGL(k_finalize, dim3(256), dim3(256), 0, st, n_dev, recs, partials, counters,
-                       out_price, out_se);
+                       out_price, out_se, host_flags);
     return hipGetLastError();
 }
 
