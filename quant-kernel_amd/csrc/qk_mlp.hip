@@ -59,24 +59,27 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const __bf16* __restrict__ A, i
     }
 }
 
-// Fused inference: the four hidden layers and the output layer of 64 rows
-// per workgroup in one launch.  Waves 2r and 2r+1 share rows 32r .. 32r+31
-// of the tile and compute output columns 0-127 and 128-255 of a layer for
-// them (4 MFMA tiles of 32x32 each).  The layer's activations stay in LDS as
-// BF16 (the A operands of the next layer); the weights of a layer pass
-// through LDS in chunks of kInfKc inputs, staged once per workgroup, and
-// each thread loads its row of the next chunk (the next layer's first after
-// a layer's last) into registers while the current chunk's MFMAs run.  At
-// about 55 KB of LDS two workgroups share a CU, so one wave's MFMAs overlap
-// another's GELU epilogue.  The hidden activations equal those of k_gemm_nt
-// + k_act_forward bitwise (same MFMA sequence over K, same bias and GELU);
-// the output dot product is summed in another order.
-constexpr int kInfRows = 64;
+// Fused inference: the four hidden layers and the output layer of 256 rows
+// per workgroup in one launch.  Waves 2r and 2r+1 (r = 0..7) share rows
+// 32r .. 32r+31 of the tile and compute output columns 0-127 and 128-255 of
+// a layer for them (4 MFMA tiles of 32x32 each).  The layer's activations
+// stay in LDS as BF16 (the A operands of the next layer); the weights of a
+// layer pass through LDS in chunks of kInfKc inputs, staged once per
+// workgroup, and the threads load the next chunk (the next layer's first
+// after a layer's last) into registers while the current chunk's MFMAs run.
+// At about 155 KB of LDS one workgroup of 16 waves fills a CU; a tile of 256
+// rows stages each weight once per 256 rows (the network stage of 1M
+// queries took 1.78, 1.64 and 1.12 ms at 64, 128 and 256 rows).  The hidden
+// activations equal those of k_gemm_nt + k_act_forward bitwise (same MFMA
+// sequence over K, same bias and GELU); the output dot product is summed in
+// another order.
+constexpr int kInfRows = 256;              // rows per workgroup: 32 per pair of waves
+constexpr int kInfThreads = kInfRows * 4;  // waves (row block, column half)
 constexpr int kInfKc = 32;  // every layer's K is a multiple of 32
 constexpr int kActStride = kH + 8;  // bf16; the pad spreads rows over the LDS banks
 constexpr int kWStride = kInfKc + 8;
 
-__global__ __launch_bounds__(256) void k_mlp_infer(const __bf16* __restrict__ X,
+__global__ __launch_bounds__(kInfThreads) void k_mlp_infer(const __bf16* __restrict__ X,
                                                    const __bf16* __restrict__ wbf,
                                                    const float* __restrict__ params,
                                                    float* __restrict__ y, int M) {
@@ -91,13 +94,19 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const __bf16* __restrict__ X,
     float ydot[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) ydot[r] = 0.0f;
-    // thread n's row of weight chunk c of layer l: Wt_l[n][32c .. 32c + 32)
-    bf16x8 pre[kInfKc / 8];
+    // weight chunk c of layer l, Wt_l[n][32c .. 32c + 32) for the kH rows n,
+    // as kH * 4 vectors of 8 bf16: thread t stages vectors
+    // t * kPer .. t * kPer + kPer - 1 (vector v: row v / 4, columns 8 (v % 4))
+    constexpr int kVec = kInfKc / 8;
+    constexpr int kPer = kH * kVec / kInfThreads;
+    bf16x8 pre[kPer];
     auto load_chunk = [&](int l, int c) {
-        const bf16x8* src = reinterpret_cast<const bf16x8*>(
-            wbf + kBfWt[l] + size_t(threadIdx.x) * kK[l] + c * kInfKc);
 #pragma unroll
-        for (int q = 0; q < kInfKc / 8; ++q) pre[q] = src[q];
+        for (int i = 0; i < kPer; ++i) {
+            const int v = int(threadIdx.x) * kPer + i;
+            pre[i] = *reinterpret_cast<const bf16x8*>(wbf + kBfWt[l] + size_t(v / kVec) * kK[l] +
+                                                       c * kInfKc + (v % kVec) * 8);
+        }
     };
     load_chunk(0, 0);
     for (int l = 0; l < kHidden; ++l) {
@@ -108,10 +117,10 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const __bf16* __restrict__ X,
         for (int c = 0; c < nch; ++c) {
             // the previous chunk (and, at c = 0, the previous epilogue) done
             __syncthreads();
-            {
-                bf16x8* dst = reinterpret_cast<bf16x8*>(wch + threadIdx.x * kWStride);
 #pragma unroll
-                for (int q = 0; q < kInfKc / 8; ++q) dst[q] = pre[q];
+            for (int i = 0; i < kPer; ++i) {
+                const int v = int(threadIdx.x) * kPer + i;
+                *reinterpret_cast<bf16x8*>(wch + (v / kVec) * kWStride + (v % kVec) * 8) = pre[i];
             }
             __syncthreads();
             if (c + 1 < nch) load_chunk(l, c + 1);
@@ -351,7 +360,7 @@ hipError_t gemm_nt(hipStream_t st, const void* A, int lda, const void* B, int ld
 hipError_t infer(hipStream_t st, const void* X, const void* wbf, const float* params, float* y,
                  int M) {
     if (M <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp_infer, dim3((M + kInfRows - 1) / kInfRows), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_mlp_infer, dim3((M + kInfRows - 1) / kInfRows), dim3(kInfThreads), 0, st,
                        static_cast<const __bf16*>(X), static_cast<const __bf16*>(wbf), params, y, M);
     return hipGetLastError();
 }
