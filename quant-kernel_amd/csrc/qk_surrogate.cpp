@@ -1,6 +1,6 @@
 // qk_surrogate.cpp - Heston price surrogate: GPU training with the project's
 // own MFMA forward/backward kernels and Adam, GPU inference, a weights file,
-// and pricing with plain Monte Carlo as the fallback and the spot-check judge.
+// and pricing with plain Monte Carlo as the fallback and as the spot check of its error.
 //
 // Targets are price/S of European calls from the Heston full-truncation Euler
 // kernel (256 steps).  Features are (log K/S, T, r, kappa, theta, sigma, rho,
