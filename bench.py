@@ -451,12 +451,13 @@ def bench_c3(args, qk, qka, dev, name, cus):
         log(f"step done: device {times[-1]:.2f} ms")
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     dev_ms = float(np.mean(times))
-    # algorithmic HBM bytes per path: the forward pass writes S at every date
-    # (4 B x dates); each of the dates - 1 regression dates then reads S twice
-    # (its regression sums, and the exercise rule once the fit is known) and
-    # reads and writes the FP32 cashflow: 16 B per path-date
+    # algorithmic HBM bytes per path: the forward pass writes S at dates
+    # 1 .. N-1 and the terminal cashflow (4 B x dates); the sweeps of dates
+    # N-1 .. 2 read S twice (S_j for the exercise rule, S_{j-1} for the next
+    # regression) and read and write the FP32 cashflow (16 B per path-date);
+    # the sweep of date 1 reads S_1 and the cashflow (8 B): 976 B per path
     paths, dates = 100000, 50
-    byts = float(n) * paths * (4 * dates + 16 * (dates - 1))
+    byts = float(n) * paths * (4 * dates + 16 * (dates - 2) + 8)
     ach = byts / (dev_ms * 1e-3) / 1e9
     res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
                config={"workload": f"C3: {n} American puts, Longstaff-Schwartz, {paths} paths x {dates} dates, basis "

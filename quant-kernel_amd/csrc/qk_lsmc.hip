@@ -2,7 +2,7 @@
 //
 // Grid: (path tile, option).  A tile is kLsmcBlock lanes x kLsmcPerLane
 // paths of one option.  Per chunk of options the host launches
-//   k_lsmc_forward           generate paths, store S_1..S_N date-major
+//   k_lsmc_forward           generate paths, store S_1..S_{N-1} date-major
 //                            (S[(j-1)*P + p], FP32 in production), set the
 //                            discounted terminal cashflow and accumulate the
 //                            normal-equation sums of date N-1;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kLsmcBlock) void k_lsmc_forward(const LsmcRec* __re
                     sprev = s;
 #pragma unroll
                     for (int e = 0; e < V; ++e) advance(s.v[e], x.v[e], z[e][c]);
-                    s.store(S + int64_t(j - 1) * P + pb);
+                    if (j < N) s.store(S + int64_t(j - 1) * P + pb);  // no sweep reads S_N
                 }
             }
             Vec out;
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kLsmcBlock) void k_lsmc_forward(const LsmcRec* __re
                 if (j > N) break;
                 sprev = s;
                 advance(s, x, z[c]);
-                S[int64_t(j - 1) * P + p] = s;
+                if (j < N) S[int64_t(j - 1) * P + p] = s;  // no sweep reads S_N
             }
         }
         const double c = finish(s, sprev);
