@@ -178,11 +178,21 @@ int32_t heston_lr_batch(const double* spot, const double* strike, const double* 
 }
 
 // Quasi-MC has no seed in the reference's signature; the engine takes one
-// per element and ignores it for point sets.
+// per element and ignores it for point sets.  The seed vector is allocated
+// inside guard so a failed allocation becomes an error code.
 int32_t qmc_batch(int32_t method, const double* spot, const double* strike, const double* t,
                   const double* vol, const double* r, const double* q, const int32_t* ot,
                   const int32_t* paths, int32_t n, double* out, double* se) {
-    std::vector<uint64_t> seed(static_cast<size_t>(n), 0);
+    std::vector<uint64_t> seed;
+    std::string err;
+    const int32_t rc = guard(err, [&] {
+        seed.assign(static_cast<size_t>(n), 0);
+        return QK_OK;
+    });
+    if (rc != QK_OK) {
+        set_msg(err);
+        return rc;
+    }
     return gbm_batch(method, spot, strike, t, vol, r, q, ot, paths, nullptr, seed.data(), n, out,
                      se);
 }

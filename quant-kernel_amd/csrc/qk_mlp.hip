@@ -78,6 +78,18 @@ constexpr int kInfThreads = kInfRows * 4;  // waves (row block, column half)
 constexpr int kInfKc = 32;  // every layer's K is a multiple of 32
 constexpr int kActStride = kH + 8;  // bf16; the pad spreads rows over the LDS banks
 constexpr int kWStride = kInfKc + 8;
+// The wave layout assumes 16 waves: 8 row blocks of 32 rows x 2 column
+// halves of 4 MFMA tiles of 32 columns.
+static_assert(kInfRows == 8 * 32 && kInfThreads == 16 * 64, "k_mlp_infer: 16 waves of 32 rows");
+static_assert(kH == 2 * 4 * 32, "k_mlp_infer: two column halves of four 32-wide MFMA tiles");
+static_assert(kInPad % kInfKc == 0 && kH % kInfKc == 0, "k_mlp_infer: K must be whole chunks");
+// Every thread stages the same whole number of 8-wide weight vectors per chunk.
+static_assert((kH * (kInfKc / 8)) % kInfThreads == 0 && kH * (kInfKc / 8) / kInfThreads >= 1,
+              "k_mlp_infer: weight chunk must split evenly over the threads");
+// Static LDS of the kernel (activations, weight chunk, half dot products)
+// within the 160 KB of a gfx950 CU.
+static_assert(kInfRows * kActStride * 2 + kH * kWStride * 2 + kInfRows * 4 <= 160 * 1024,
+              "k_mlp_infer: LDS over 160 KB");
 
 __global__ __launch_bounds__(kInfThreads) void k_mlp_infer(const __bf16* __restrict__ X,
                                                    const __bf16* __restrict__ wbf,

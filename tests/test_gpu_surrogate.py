@@ -48,7 +48,9 @@ def test_fused_inference_matches_training_forward(qk):
     product summed in another order)."""
     p = mo.init_params(9)
     qk.sur_set_params(p)
-    for n in (1, 127, 128, 129, 1000):  # partial and full 128-row tiles
+    # partial, full and one-past-full 256-row workgroup tiles (k_mlp_infer), and a
+    # batch spanning several tiles with a ragged last one
+    for n in (1, 127, 255, 256, 257, 513, 1000):
         f = _box(n, 10 + n)
         t = np.random.default_rng(n).random(n) * 0.3
         y = qk.sur_predict(f) - mo.control(f)
