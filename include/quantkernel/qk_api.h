@@ -352,6 +352,29 @@ QK_EXPORT int32_t qk_mcm_mixed_price_batch_device(const int32_t* model,
         const double* rho, const int32_t* option_type, const int32_t* paths, const int32_t* steps,
         const uint64_t* seed, int32_t n, double* out_prices, double* out_stderr);
 
+/* Allocation-free variant of qk_mcm_mixed_price_batch_device, with the same
+ * arguments and results.  It has no counterpart in the reference: it is for
+ * callers that cannot allocate device memory during the call (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * calls no hipMalloc, hipFree, hipHostMalloc or graph instantiation: its work
+ * buffers live in one fixed device array of the library (about 44.5 MiB,
+ * held by every device the library runs on until the process ends).  Limits:
+ * at most 65535 options (larger n returns QK_ERR_BAD_SIZE before any device
+ * work) and at most 2^19 path tiles of the current tile shape (a tile is 256
+ * lanes x the stepwise or terminal lane_units of qk_mc_set_shape; a batch
+ * needing more runs no tile, its outputs are NaN and the call returns
+ * QK_ERR_INVALID_INPUT).  All calls in a process share that array; the
+ * library's device lock serialises them, and each call finishes its device
+ * work before it returns.  The device context (stream, events, the array's
+ * device address) is set up by the library's first call, which may be this
+ * one. */
+QK_EXPORT int32_t qk_mcm_mixed_price_batch_device_noalloc(const int32_t* model,
+        const double* spot, const double* strike, const double* t, const double* vol,
+        const double* r, const double* q,
+        const double* v0, const double* kappa, const double* theta, const double* sigma,
+        const double* rho, const int32_t* option_type, const int32_t* paths, const int32_t* steps,
+        const uint64_t* seed, int32_t n, double* out_prices, double* out_stderr);
+
 /* Standard MC (antithetic terminal estimator) on device-resident inputs and
  * outputs, with standard errors; QK_RNG_PHILOX only.  Equal to
  * qk_mcm_standard_monte_carlo_price_batch_se for the same inputs.

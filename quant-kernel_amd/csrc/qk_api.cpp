@@ -304,6 +304,51 @@ namespace qk {
 void set_last_error(const char* msg) { set_msg(msg); }
 }  // namespace qk
 
+namespace {
+
+// Shared by the two device-resident mixed entry points: pointer and size
+// checks, then the BatchIn of the device arrays.
+bool mixed_device_args(const int32_t* model, const double* spot, const double* strike,
+                       const double* t, const double* vol, const double* r, const double* q,
+                       const double* v0, const double* kappa, const double* theta,
+                       const double* sigma, const double* rho, const int32_t* option_type,
+                       const int32_t* paths, const int32_t* steps, const uint64_t* seed,
+                       int32_t n, double* out_prices, double* out_stderr, qk::BatchIn& bi,
+                       int32_t& rc) {
+    if (!check_ptrs({QK_P(model), QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
+                     QK_P(v0), QK_P(kappa), QK_P(theta), QK_P(sigma), QK_P(rho),
+                     QK_P(option_type), QK_P(paths), QK_P(steps), QK_P(seed), QK_P(out_prices),
+                     QK_P(out_stderr)})) {
+        rc = QK_ERR_NULL_PTR;
+        return false;
+    }
+    if (!check_n(n)) {
+        rc = QK_ERR_BAD_SIZE;
+        return false;
+    }
+    bi = qk::BatchIn{};
+    bi.method = model;
+    bi.method_is_model = 1;
+    bi.spot = spot;
+    bi.strike = strike;
+    bi.t = t;
+    bi.vol = vol;
+    bi.r = r;
+    bi.q = q;
+    bi.v0 = v0;
+    bi.kappa = kappa;
+    bi.theta = theta;
+    bi.sigma = sigma;
+    bi.rho = rho;
+    bi.ot = option_type;
+    bi.paths = paths;
+    bi.steps = steps;
+    bi.seed = seed;
+    return true;
+}
+
+}  // namespace
+
 extern "C" {
 
 // ---- ABI negotiation and errors (reference qk_api.cpp:94-117) -------------
@@ -915,33 +960,35 @@ int32_t qk_mcm_mixed_price_batch_device(const int32_t* model, const double* spot
                                         const int32_t* option_type, const int32_t* paths,
                                         const int32_t* steps, const uint64_t* seed, int32_t n,
                                         double* out_prices, double* out_stderr) {
-    if (!check_ptrs({QK_P(model), QK_P(spot), QK_P(strike), QK_P(t), QK_P(vol), QK_P(r), QK_P(q),
-                     QK_P(v0), QK_P(kappa), QK_P(theta), QK_P(sigma), QK_P(rho),
-                     QK_P(option_type), QK_P(paths), QK_P(steps), QK_P(seed), QK_P(out_prices),
-                     QK_P(out_stderr)}))
-        return QK_ERR_NULL_PTR;
-    if (!check_n(n)) return QK_ERR_BAD_SIZE;
-    qk::BatchIn bi{};
-    bi.method = model;
-    bi.method_is_model = 1;
-    bi.spot = spot;
-    bi.strike = strike;
-    bi.t = t;
-    bi.vol = vol;
-    bi.r = r;
-    bi.q = q;
-    bi.v0 = v0;
-    bi.kappa = kappa;
-    bi.theta = theta;
-    bi.sigma = sigma;
-    bi.rho = rho;
-    bi.ot = option_type;
-    bi.paths = paths;
-    bi.steps = steps;
-    bi.seed = seed;
+    qk::BatchIn bi;
+    int32_t rc = QK_OK;
+    if (!mixed_device_args(model, spot, strike, t, vol, r, q, v0, kappa, theta, sigma, rho,
+                           option_type, paths, steps, seed, n, out_prices, out_stderr, bi, rc))
+        return rc;
     std::string err;
-    const int32_t rc = guard(err, [&] {
-        return qk::run_mc_batch_device(bi, n, out_prices, out_stderr, &err);
+    rc = guard(err, [&] { return qk::run_mc_batch_device(bi, n, out_prices, out_stderr, &err); });
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
+int32_t qk_mcm_mixed_price_batch_device_noalloc(const int32_t* model, const double* spot,
+                                                const double* strike, const double* t,
+                                                const double* vol, const double* r,
+                                                const double* q, const double* v0,
+                                                const double* kappa, const double* theta,
+                                                const double* sigma, const double* rho,
+                                                const int32_t* option_type, const int32_t* paths,
+                                                const int32_t* steps, const uint64_t* seed,
+                                                int32_t n, double* out_prices,
+                                                double* out_stderr) {
+    qk::BatchIn bi;
+    int32_t rc = QK_OK;
+    if (!mixed_device_args(model, spot, strike, t, vol, r, q, v0, kappa, theta, sigma, rho,
+                           option_type, paths, steps, seed, n, out_prices, out_stderr, bi, rc))
+        return rc;
+    std::string err;
+    rc = guard(err, [&] {
+        return qk::run_mc_batch_device_noalloc(bi, n, out_prices, out_stderr, &err);
     });
     if (rc != QK_OK) set_msg(err);
     return rc;

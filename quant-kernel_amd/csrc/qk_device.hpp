@@ -109,6 +109,12 @@ struct Device {
     bool graphs_ok = true;        // capture and instantiation work on this runtime
     bool graphs_enabled = true;   // caller switch (qk_mc_set_graphs)
     int64_t graph_launches = 0;   // replays + first launches, for diagnostics
+
+    // Device address of this device's copy of the allocation-free entry
+    // point's fixed scratch array (hipGetSymbolAddress at set-up), or null
+    // with noalloc_err saying why.
+    char* noalloc_scratch = nullptr;
+    std::string noalloc_err;
 };
 
 // Returns the context (creating it on first use) or null with *err set.

@@ -30,6 +30,25 @@ namespace {
 
 std::atomic<int32_t> g_rng_mode{QK_RNG_PHILOX};
 
+// Fixed device scratch of the allocation-free MC entry point
+// (run_mc_batch_device_noalloc): option records, per-option tile counts and
+// offsets, the pipeline's counters and the tile partials, each 256-byte
+// aligned.  Every device the library runs on holds its own copy until the
+// process ends.  In host code the array's name is not a device address: the
+// address comes from hipGetSymbolAddress in init_device, once per device.
+constexpr size_t kNaRecsOff = 0;
+constexpr size_t kNaNtilesOff = kNaRecsOff + size_t(65536) * sizeof(OptionRec);
+constexpr size_t kNaStartOff = kNaNtilesOff + size_t(65536) * 4;
+constexpr size_t kNaCountersOff = kNaStartOff + size_t(65536 + 64) * 4;
+constexpr size_t kNaPartialsOff = kNaCountersOff + 256;
+constexpr size_t kNaBytes = kNaPartialsOff + size_t(kNoAllocMaxTiles) * sizeof(Partial);
+static_assert(kNaNtilesOff % 256 == 0 && kNaStartOff % 256 == 0 && kNaCountersOff % 256 == 0 &&
+                  kNaPartialsOff % 256 == 0,
+              "scratch sections are 256-byte aligned");
+static_assert(kNoAllocMaxOptions < 65536, "options beyond the record section");
+
+static __device__ __align__(256) char scratch[kNaBytes];
+
 Device* g_dev = nullptr;
 }  // namespace
 void apply_cached_shapes(Device* d);
@@ -77,6 +96,16 @@ void init_device() {
         g_init_err = std::string("stream/event creation: ") + hipGetErrorString(e);
         delete d;
         return;
+    }
+    // this device's copy of the allocation-free entry point's scratch
+    void* sp = nullptr;
+    e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && sp) {
+        d->noalloc_scratch = static_cast<char*>(sp);
+    } else {
+        (void)hipGetLastError();
+        d->noalloc_err = std::string("hipGetSymbolAddress(scratch): ") +
+                         (e != hipSuccess ? hipGetErrorString(e) : "null address");
     }
     const int per_cu = std::max(1, env_int("QK_GRID_PER_CU", 8));
     d->shape_step.lane_units = std::max(1, env_int("QK_LANE_PATHS", 8));
@@ -128,16 +157,33 @@ int32_t* host_flags_ptr(void* pinned, size_t offset) {
 // batch size is n, which prepare stores in counters[8] for the later kernels.
 // host_flags (device address of pinned host memory, or null) receives the
 // scan's counters[0..3] from the finalizer.
-int32_t launch_pipeline(Device* d, const BatchIn& in, int32_t n, const int32_t* n_dev,
-                        const int64_t* stream_off, int32_t family, int32_t mode,
-                        const void* stream, double* out_price, double* out_se,
+// Work buffers of one pipeline run: the context's grow-only buffers, or the
+// fixed device scratch of the allocation-free entry point.
+struct PipeBufs {
+    OptionRec* recs;
+    int32_t* ntiles;
+    int32_t* tile_start;
+    int32_t* counters;  // at least 9 ints
+    void* scan;         // block sums of the multi-pass scan (n > 65535 only)
+    Partial* partials;
+};
+
+PipeBufs context_bufs(Device* d) {
+    return PipeBufs{static_cast<OptionRec*>(d->recs.p), static_cast<int32_t*>(d->ntiles.p),
+                    static_cast<int32_t*>(d->tile_start.p), static_cast<int32_t*>(d->counters.p),
+                    d->scan.p, static_cast<Partial*>(d->partials.p)};
+}
+
+int32_t launch_pipeline(Device* d, const PipeBufs& bufs, const BatchIn& in, int32_t n,
+                        const int32_t* n_dev, const int64_t* stream_off, int32_t family,
+                        int32_t mode, const void* stream, double* out_price, double* out_se,
                         std::string* err, bool timed = true, int32_t capacity = 0x7fffffff,
                         int32_t* host_flags = nullptr) {
-    OptionRec* recs = static_cast<OptionRec*>(d->recs.p);
-    int32_t* ntiles = static_cast<int32_t*>(d->ntiles.p);
-    int32_t* tstart = static_cast<int32_t*>(d->tile_start.p);
-    int32_t* ctr = static_cast<int32_t*>(d->counters.p);
-    Partial* parts = static_cast<Partial*>(d->partials.p);
+    OptionRec* recs = bufs.recs;
+    int32_t* ntiles = bufs.ntiles;
+    int32_t* tstart = bufs.tile_start;
+    int32_t* ctr = bufs.counters;
+    Partial* parts = bufs.partials;
     const SimShape& shp = family == FAM_TERMINAL ? d->shape_term : d->shape_step;
     int32_t* n_slot = n_dev ? nullptr : ctr + 8;
     if (!n_dev) n_dev = n_slot;
@@ -145,7 +191,7 @@ int32_t launch_pipeline(Device* d, const BatchIn& in, int32_t n, const int32_t* 
     QK_HIP(launch_prepare(d->stream, in, n_slot ? nullptr : n_dev, n, n_slot, stream_off,
                           tile_groups_terminal(d->shape_term), tile_units_stepwise(d->shape_step),
                           recs, ntiles));
-    QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, d->scan.p, capacity));
+    QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, bufs.scan, capacity));
     if (timed) QK_HIP(hipEventRecord(d->ev[1], d->stream));
     QK_HIP(launch_simulate(d->stream, family, mode, shp, recs, tstart, n_dev, ctr, parts, stream));
     if (timed) QK_HIP(hipEventRecord(d->ev[2], d->stream));
@@ -495,8 +541,8 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
             QK_HIP(hipMemcpyAsync(d->in.p, d->pin_in.p, in_copy, hipMemcpyHostToDevice, d->stream));
         // inside a graph the pipeline records no events (they are taken around it);
         // the finalizer stores the scan's counters after the outputs in pin_out
-        int32_t rc = launch_pipeline(d, bi, n, n_dev, d_off, family, sim_mode, d_stream, dprice, dse,
-                                     err, !use_graph, 0x7fffffff, hflags);
+        int32_t rc = launch_pipeline(d, context_bufs(d), bi, n, n_dev, d_off, family, sim_mode,
+                                     d_stream, dprice, dse, err, !use_graph, 0x7fffffff, hflags);
         if (rc != QK_OK) return rc;
         QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, ns * 16, hipMemcpyDeviceToHost, d->stream));
         return QK_OK;
@@ -619,8 +665,9 @@ int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
         // n travels as a kernel argument; the finalizer stores the scan's
         // counters in pin_out: the graph holds only the four kernels
         auto enqueue = [&]() -> int32_t {
-            return launch_pipeline(d, in, n, nullptr, nullptr, family, SIM_PHILOX_F32, nullptr,
-                                   out_price_dev, out_se_dev, err, !use_graph, cap, hflags);
+            return launch_pipeline(d, context_bufs(d), in, n, nullptr, nullptr, family,
+                                   SIM_PHILOX_F32, nullptr, out_price_dev, out_se_dev, err,
+                                   !use_graph, cap, hflags);
         };
         if (use_graph) {
             const SimShape& st = d->shape_step;
@@ -659,6 +706,59 @@ int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
     }
     if (err) *err = "tile partials could not be sized for the batch";
     return QK_ERR_RUNTIME_INIT;
+}
+
+int32_t run_mc_batch_device_noalloc(const BatchIn& in, int32_t n, double* out_price_dev,
+                                    double* out_se_dev, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    if (n > kNoAllocMaxOptions) {
+        if (err)
+            *err = "batch of " + std::to_string(n) +
+                   " options exceeds the allocation-free entry point's maximum of " +
+                   std::to_string(kNoAllocMaxOptions);
+        return QK_ERR_BAD_SIZE;
+    }
+    if (get_rng_mode() != QK_RNG_PHILOX) {
+        if (err) *err = "device-resident batches support QK_RNG_PHILOX only";
+        return QK_ERR_INVALID_INPUT;
+    }
+    if (!d->noalloc_scratch) {
+        if (err) *err = d->noalloc_err;
+        return QK_ERR_RUNTIME_INIT;
+    }
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+    char* base = d->noalloc_scratch;
+    // n <= 65535 always takes the one-pass scan, which needs no block sums
+    const PipeBufs bufs{reinterpret_cast<OptionRec*>(base + kNaRecsOff),
+                        reinterpret_cast<int32_t*>(base + kNaNtilesOff),
+                        reinterpret_cast<int32_t*>(base + kNaStartOff),
+                        reinterpret_cast<int32_t*>(base + kNaCountersOff), nullptr,
+                        reinterpret_cast<Partial*>(base + kNaPartialsOff)};
+    const int32_t family =
+        (!in.method && is_terminal_method(in.uniform_method)) ? FAM_TERMINAL : FAM_STEPWISE;
+    // direct launches (no graph: instantiating one allocates); a batch that
+    // needs more tiles than the scratch holds runs no tile and gets NaN
+    int32_t rc = launch_pipeline(d, bufs, in, n, nullptr, nullptr, family, SIM_PHILOX_F32, nullptr,
+                                 out_price_dev, out_se_dev, err, true, kNoAllocMaxTiles, nullptr);
+    if (rc != QK_OK) return rc;
+    int32_t flags[4] = {0, 0, 0, 0};
+    QK_HIP(hipMemcpyAsync(flags, bufs.counters, sizeof(flags), hipMemcpyDeviceToHost, d->stream));
+    QK_HIP(hipStreamSynchronize(d->stream));
+    record_timing(d);
+    if (flags[2] == 2) {
+        if (err) *err = "batch too large: more than 2^31-1 path tiles";
+        return QK_ERR_INVALID_INPUT;
+    }
+    if (flags[2] != 0) {
+        if (err)
+            *err = "batch needs " + std::to_string(flags[3]) +
+                   " path tiles; the allocation-free entry point holds at most " +
+                   std::to_string(kNoAllocMaxTiles) + " (outputs are NaN)";
+        return QK_ERR_INVALID_INPUT;
+    }
+    return QK_OK;
 }
 
 // Longstaff-Schwartz batch: host validation (longstaff_schwartz.cpp:50-55),

@@ -35,6 +35,18 @@ int32_t run_mc_batch(const HostBatch& hb, int32_t n, double* out_price, double* 
 int32_t run_mc_batch_device(const BatchIn& in, int32_t n, double* out_price_dev,
                             double* out_se_dev, std::string* err);
 
+// Limits of the allocation-free variant's fixed device scratch.
+constexpr int32_t kNoAllocMaxOptions = 65535;
+constexpr int32_t kNoAllocMaxTiles = 1 << 19;  // 2^19 tiles x 64-byte partials = 32 MiB
+
+// run_mc_batch_device without any device or pinned allocation: its work
+// buffers live in a fixed device array of the library (about 44.5 MiB per
+// device).  Batches of more than kNoAllocMaxOptions options, or that need
+// more than kNoAllocMaxTiles path tiles, are rejected.  Launches directly
+// (no launch graph).
+int32_t run_mc_batch_device_noalloc(const BatchIn& in, int32_t n, double* out_price_dev,
+                                    double* out_se_dev, std::string* err);
+
 struct Device;
 
 // The device MC pipeline on device-resident inputs; the caller holds d->mu

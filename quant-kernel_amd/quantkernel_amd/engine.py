@@ -138,6 +138,7 @@ _SIGNATURES = {
     "qk_cf_heston_price_cf_batch_device": (_I, [ct.c_void_p] * 13 + [_I, ct.c_void_p]),
     "qk_mcm_mixed_price_batch": (_I, [_PI] + [_PD] * 11 + [_PI, _PI, _PI, _PU, _I, _PD, _PD]),
     "qk_mcm_mixed_price_batch_device": (_I, [ct.c_void_p] * 16 + [_I, ct.c_void_p, ct.c_void_p]),
+    "qk_mcm_mixed_price_batch_device_noalloc": (_I, [ct.c_void_p] * 16 + [_I, ct.c_void_p, ct.c_void_p]),
     "qk_mcm_standard_monte_carlo_price_batch_device": (_I, [ct.c_void_p] * 9 + [_I, ct.c_void_p, ct.c_void_p]),
     "qk_mc_set_rng_mode": (_I, [_I]),
     "qk_mc_get_rng_mode": (_I, []),
@@ -754,5 +755,15 @@ class QuantKernel:
         rc = self._lib.qk_mcm_mixed_price_batch_device(*args, int(n), ct.c_void_p(int(out_price_ptr)),
                                                        ct.c_void_p(int(out_se_ptr)))
         self._check(rc, "qk_mcm_mixed_price_batch_device")
+
+    def mixed_price_batch_device_noalloc(self, n: int, ptrs: dict, out_price_ptr: int, out_se_ptr: int) -> None:
+        """Allocation-free variant of mixed_price_batch_device (fixed device scratch; at most
+        65535 options and 2^19 path tiles)."""
+        order = ("model", "spot", "strike", "t", "vol", "r", "q", "v0", "kappa", "theta", "sigma", "rho",
+                 "option_type", "paths", "steps", "seed")
+        args = [ct.c_void_p(int(ptrs[k])) for k in order]
+        rc = self._lib.qk_mcm_mixed_price_batch_device_noalloc(*args, int(n), ct.c_void_p(int(out_price_ptr)),
+                                                               ct.c_void_p(int(out_se_ptr)))
+        self._check(rc, "qk_mcm_mixed_price_batch_device_noalloc")
 
 
