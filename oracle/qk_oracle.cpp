@@ -139,20 +139,23 @@ double philox_path_normal(uint64_t seed, uint32_t p, uint32_t d) {
 }
 
 // Three-pair layout of one Philox block (counter in[], key = seed): slot s
-// takes u1 = (2m + 1) 2^-24 with m the low 23 bits of word s; the angle
-// fraction is the low 23 bits of w3 (slot 0), or 15 bits built from bits
-// 24-30 of w0 and 24-31 of w1 (slot 1) / of w2 and w3 (slot 2), over 2^23.
-// (z1, z2) = sqrt(-2 ln u1) (cos 2 pi f, sin 2 pi f) in FP64.
+// takes u1 = (2m + 1) 2^-24 with m the low 23 bits of word s, and a 12-bit
+// angle index a: bits 3-14 of w3 (slot 0); bits 19-23 of w3 below bits 24-30
+// of w0 (slot 1); bits 27-31 of w1 below bits 24-30 of w2 (slot 2).
+// (z1, z2) = sqrt(-2 ln u1) (cos 2 pi a / 4096, sin 2 pi a / 4096) in FP64
+// (the GPU reads FP32 roundings of the same cosine and sine from a table).
+uint32_t slot_angle_index(const uint32_t w[4], uint32_t s) {
+    if (s == 0) return (w[3] >> 3) & 0xFFFu;
+    if (s == 1) return ((w[3] >> 19) & 0x1Fu) | ((w[0] >> 24) & 0x7Fu) << 5;
+    return ((w[1] >> 27) & 0x1Fu) | ((w[2] >> 24) & 0x7Fu) << 5;
+}
+
 void philox_slot_pair(const uint32_t in[4], uint64_t seed, uint32_t s, double* z1, double* z2) {
     const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
     uint32_t w[4];
     philox10(in, key, w);
     const double u1 = (2.0 * double(w[s] & 0x007FFFFFu) + 1.0) * 0x1p-24;
-    uint32_t fb;
-    if (s == 0) fb = w[3] & 0x007FFFFFu;
-    else if (s == 1) fb = ((w[0] >> 24) & 0x7Fu) << 16 | (w[1] >> 24) << 8;
-    else fb = ((w[2] >> 24) & 0x7Fu) << 16 | (w[3] >> 24) << 8;
-    const double f = double(fb) * (1.0 / 8388608.0);
+    const double f = double(slot_angle_index(w, s)) * (1.0 / 4096.0);
     const double rad = std::sqrt(-2.0 * std::log(u1));
     const double ang = 6.283185307179586476925286766559 * f;
     *z1 = rad * std::cos(ang);

@@ -97,6 +97,12 @@ void init_device() {
         delete d;
         return;
     }
+    e = upload_trig_table();
+    if (e != hipSuccess) {
+        g_init_err = std::string("angle table upload: ") + hipGetErrorString(e);
+        delete d;
+        return;
+    }
     // this device's copy of the allocation-free entry point's scratch
     void* sp = nullptr;
     e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));

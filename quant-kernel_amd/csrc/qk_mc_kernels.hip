@@ -19,6 +19,8 @@
 // Reference loops restated: mc_engine.h:32-53 (antithetic terminal),
 // mc_engine.h:84-103 (stepwise), mc_engine.h:113-136 (2-D stepwise),
 // local_vol_monte_carlo.cpp:20-48; step models model_concepts.h:93-266.
+#include <cmath>
+
 #include "qk_common.hpp"
 #include "qk_kernels.hpp"
 #include "qk_rng.hpp"
@@ -75,7 +77,8 @@ struct Draws {
 // Production: Philox blocks keyed by the option seed (wave-uniform key).
 struct PhiloxSrc {
     uint32_t k0, k1;
-    uint32_t tag = 0;  // fourth counter word of path blocks (the MLMC level)
+    uint32_t tag = 0;               // fourth counter word of path blocks (the MLMC level)
+    const float2* trig = nullptr;   // LDS angle table of the three-pair layout
     __device__ __forceinline__ Quad<float> path_block(uint32_t p, uint32_t b, int32_t) const {
         const Normal4 n = philox_normals(p, b, tag, k0, k1);
         return Quad<float>{{n.z0, n.z1, n.z2, n.z3}};
@@ -91,7 +94,7 @@ struct PhiloxSrc {
     // normals k = 6b .. 6b+5 of a terminal estimator: the three pairs of
     // block (b, 0, 0, 1)
     __device__ __forceinline__ Draws<float, 6> term_normals6(uint32_t b) const {
-        const Normal6 n = philox_normals6(philox4x32_10(b, 0u, 1u, k0, k1));
+        const Normal6 n = philox_normals6(philox4x32_10(b, 0u, 1u, k0, k1), trig);
         return Draws<float, 6>{{n.z[0], n.z[1], n.z[2], n.z[3], n.z[4], n.z[5]}};
     }
     __device__ __forceinline__ float term_draw(uint32_t k) const {
@@ -129,6 +132,12 @@ struct StreamSrc {
     }
     __device__ __forceinline__ R term_draw(uint32_t k) const { return base[k]; }
 };
+
+// ----------------------------------------------------------------------------
+// angle table of the three-pair Philox layout: entry a = (cos, sin) of
+// 2 pi a / 4096, FP64 values rounded to FP32 on the host (upload_trig_table)
+// ----------------------------------------------------------------------------
+__device__ __align__(16) float2 g_trig[kTrigEntries];
 
 // ----------------------------------------------------------------------------
 // prepare / scan
@@ -502,24 +511,24 @@ __device__ __forceinline__ R heston_path(const Src& src, uint32_t p, int32_t ste
 // Production Heston path: the scheme of heston_path on the Philox stream,
 // rearranged to issue fewer instructions.  With Box-Muller radius
 // r = sqrt(2 ln 2) * sqrt(-log2 u1) and angle theta,
-//   z1              = r cos(theta)
-//   rho z1 + rho' z2 = r cos(theta - phi),   cos phi = rho, sin phi = rho'
+//   z1               = r cos(theta)
+//   rho z1 + rho' z2 = r (rho cos(theta) + rho' sin(theta))
 // and the log-spot is kept in units of c sqrt(2 ln 2) (c = sqrt(dt) log2e),
 // so per step: y <- max(y + (a - h v)/cK + sqrt(v) r' cos(theta), floor/cK),
-// v <- max(0, (1 - kappa dt) v + kappa theta dt + sigma sqrt(dt) sqrt(2 ln 2)
-// sqrt(v) r' cos(theta - phi)), with r' = sqrt(-log2 u1).
+// v <- max(0, (1 - kappa dt) v + kappa theta dt + sqrt(v) r' (A cos(theta) +
+// B sin(theta))), with r' = sqrt(-log2 u1), A = sigma sqrt(dt) sqrt(2 ln 2) rho
+// and B the same with rho' = sqrt(1 - rho^2).
 //
 // Heston stream: Philox block (p, j/3, 0, 0) drives steps 3(j/3) .. 3(j/3)+2
 // of path p.  Slot s of the block (words w0..w3):
-//   u1  = (2m + 1) 2^-24, m = low 23 bits of w_s (exact in FP32, in (0,1));
-//   angle fraction (revolutions) from bits disjoint from all three m:
-//     slot 0: low 23 bits of w3;
-//     slot 1: bits 24-30 of w0 and 24-31 of w1 (15 bits);
-//     slot 2: bits 24-30 of w2 and 24-31 of w3 (15 bits).
+//   u1 = (2m + 1) 2^-24, m = low 23 bits of w_s (exact in FP32, in (0,1));
+//   (cos, sin) of the 12-bit angle index of slot s from the LDS table
+//   (trig_offsets: bits disjoint from all three m).
 // One block per three steps instead of two saves a third of the Philox work;
-// the radius is capped at sqrt(48 ln 2) = 5.77 sigma.
+// the table replaces the two cosines of a step, whose transcendental issue
+// cost three FMAs each; the radius is capped at sqrt(48 ln 2) = 5.77 sigma.
 struct HestonPx {
-    float a, h, omk, kth, sgk, phi, yf, v0, ck;
+    float a, h, omk, kth, A, B, yf, v0, ck;
 };
 
 // max(x, f) for finite x, f as v_med3_f32(x, f, FLT_MAX): unlike fmaxf it
@@ -530,29 +539,33 @@ __device__ __forceinline__ float floor_max(float x, float f) {
 
 // sqrt(v) r' = sqrt(-v log2 u1): one square root per step instead of two.
 __device__ __forceinline__ void heston_px_step(const HestonPx& k, float& y, float& v, float u1,
-                                               float rev) {
+                                               float2 cs) {
     const float lg = __builtin_amdgcn_logf(u1);  // < 0
-    const float c1 = __builtin_amdgcn_cosf(rev);
-    const float c2 = __builtin_amdgcn_cosf(rev - k.phi);
     const float t = __builtin_amdgcn_sqrtf(-(v * lg));
-    y = floor_max(__builtin_fmaf(t, c1, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
-    v = __builtin_fmaxf(__builtin_fmaf(k.sgk * t, c2, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
+    y = floor_max(__builtin_fmaf(t, cs.x, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
+    const float m = __builtin_fmaf(k.B, cs.y, k.A * cs.x);
+    v = __builtin_fmaxf(__builtin_fmaf(t, m, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
 }
 
 __device__ __forceinline__ float heston_path_px(uint32_t p, int32_t steps, const HestonPx& k,
-                                                uint32_t k0, uint32_t k1) {
+                                                uint32_t k0, uint32_t k1,
+                                                const float2* trig) {
     float y = 0.0f, v = k.v0;
     const int32_t nb = steps / 3, rem = steps - 3 * nb;
     for (int32_t b = 0; b < nb; ++b) {
         const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
-        heston_px_step(k, y, v, u23(w.x), rev23(w.w));
-        heston_px_step(k, y, v, u23(w.y), rev23(top_bytes(w.x, w.y)));
-        heston_px_step(k, y, v, u23(w.z), rev23(top_bytes(w.z, w.w)));
+        const TrigOff t = trig_offsets(w);
+        const float2 c0 = trig_at(trig, t.o[0]), c1 = trig_at(trig, t.o[1]),
+                     c2 = trig_at(trig, t.o[2]);
+        heston_px_step(k, y, v, u23(w.x), c0);
+        heston_px_step(k, y, v, u23(w.y), c1);
+        heston_px_step(k, y, v, u23(w.z), c2);
     }
     if (rem) {
         const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
-        heston_px_step(k, y, v, u23(w.x), rev23(w.w));
-        if (rem > 1) heston_px_step(k, y, v, u23(w.y), rev23(top_bytes(w.x, w.y)));
+        const TrigOff t = trig_offsets(w);
+        heston_px_step(k, y, v, u23(w.x), trig_at(trig, t.o[0]));
+        if (rem > 1) heston_px_step(k, y, v, u23(w.y), trig_at(trig, t.o[1]));
     }
     return y * k.ck;  // log2(S_T / spot)
 }
@@ -563,46 +576,49 @@ __device__ __forceinline__ float heston_path_px(uint32_t p, int32_t steps, const
 // m = -2 ln 2 c1^2 for Euler and local vol folds the volatility into the
 // radius (c1 z = sqrt(m log2 u1) cos theta); Milstein keeps z (m = -2 ln 2).
 template <bool MIL>
-__device__ __forceinline__ void gbm_px_pair(float& s, float u1, float rev, float c0, float c1,
+__device__ __forceinline__ void gbm_px_pair(float& s, float u1, float2 cs, float c0, float c1,
                                             float c2, float m, bool both) {
     const float fl = 1e-12f;
     const float rad = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u1) * m);
-    const float cs = __builtin_amdgcn_cosf(rev);
     if constexpr (MIL) {
-        const float z = rad * cs;
+        const float z = rad * cs.x;
         s = q_max(fl, s * q_fma(q_fma(c2, z, c1), z, c0));
     } else {
-        s = q_max(fl, s * q_fma(rad, cs, c0));
+        s = q_max(fl, s * q_fma(rad, cs.x, c0));
     }
     if (both) {
-        const float sn = __builtin_amdgcn_sinf(rev);
         if constexpr (MIL) {
-            const float z = rad * sn;
+            const float z = rad * cs.y;
             s = q_max(fl, s * q_fma(q_fma(c2, z, c1), z, c0));
         } else {
-            s = q_max(fl, s * q_fma(rad, sn, c0));
+            s = q_max(fl, s * q_fma(rad, cs.y, c0));
         }
     }
 }
 
 template <bool MIL>
 __device__ __forceinline__ float gbm_path_px(uint32_t p, int32_t steps, float s, float c0, float c1,
-                                             float c2, uint32_t k0, uint32_t k1) {
+                                             float c2, uint32_t k0, uint32_t k1,
+                                             const float2* trig) {
     const float m = MIL ? -1.3862943611198906f : -1.3862943611198906f * c1 * c1;
     const int32_t nb = steps / 6, rem = steps - 6 * nb;
     for (int32_t b = 0; b < nb; ++b) {
         const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
-        gbm_px_pair<MIL>(s, u23(w.x), rev23(w.w), c0, c1, c2, m, true);
-        gbm_px_pair<MIL>(s, u23(w.y), rev23(top_bytes(w.x, w.y)), c0, c1, c2, m, true);
-        gbm_px_pair<MIL>(s, u23(w.z), rev23(top_bytes(w.z, w.w)), c0, c1, c2, m, true);
+        const TrigOff t = trig_offsets(w);
+        const float2 a0 = trig_at(trig, t.o[0]), a1 = trig_at(trig, t.o[1]),
+                     a2 = trig_at(trig, t.o[2]);
+        gbm_px_pair<MIL>(s, u23(w.x), a0, c0, c1, c2, m, true);
+        gbm_px_pair<MIL>(s, u23(w.y), a1, c0, c1, c2, m, true);
+        gbm_px_pair<MIL>(s, u23(w.z), a2, c0, c1, c2, m, true);
     }
     if (rem) {
         const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
-        gbm_px_pair<MIL>(s, u23(w.x), rev23(w.w), c0, c1, c2, m, rem > 1);
+        const TrigOff t = trig_offsets(w);
+        gbm_px_pair<MIL>(s, u23(w.x), trig_at(trig, t.o[0]), c0, c1, c2, m, rem > 1);
         if (rem > 2)
-            gbm_px_pair<MIL>(s, u23(w.y), rev23(top_bytes(w.x, w.y)), c0, c1, c2, m, rem > 3);
+            gbm_px_pair<MIL>(s, u23(w.y), trig_at(trig, t.o[1]), c0, c1, c2, m, rem > 3);
         if (rem > 4)
-            gbm_px_pair<MIL>(s, u23(w.z), rev23(top_bytes(w.z, w.w)), c0, c1, c2, m, false);
+            gbm_px_pair<MIL>(s, u23(w.z), trig_at(trig, t.o[2]), c0, c1, c2, m, false);
     }
     return s;
 }
@@ -637,44 +653,45 @@ __device__ __forceinline__ void heston_pair(const Src& src, uint32_t p, int32_t 
 // paths (negating both normals is negating the radius).  lg and c1 return
 // log2 u1 and cos(theta) of the step (z1 = sqrt(2 ln 2) sqrt(-lg) c1).
 __device__ __forceinline__ void heston_px_pair_step(const HestonPx& k, float& y, float& v,
-                                                    float& ya, float& va, float u1, float rev,
+                                                    float& ya, float& va, float u1, float2 cs,
                                                     float& lg, float& c1) {
     lg = __builtin_amdgcn_logf(u1);
-    c1 = __builtin_amdgcn_cosf(rev);
-    const float c2 = __builtin_amdgcn_cosf(rev - k.phi);
+    c1 = cs.x;
+    const float m = __builtin_fmaf(k.B, cs.y, k.A * cs.x);
     const float t = __builtin_amdgcn_sqrtf(-(v * lg));
     y = floor_max(__builtin_fmaf(t, c1, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
-    v = __builtin_fmaxf(__builtin_fmaf(k.sgk * t, c2, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
+    v = __builtin_fmaxf(__builtin_fmaf(t, m, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
     const float ta = __builtin_amdgcn_sqrtf(-(va * lg));
     ya = floor_max(__builtin_fmaf(-ta, c1, ya + __builtin_fmaf(-k.h, va, k.a)), k.yf);
-    va = __builtin_fmaxf(__builtin_fmaf(-(k.sgk * ta), c2, __builtin_fmaf(k.omk, va, k.kth)),
-                         0.0f);
+    va = __builtin_fmaxf(__builtin_fmaf(-ta, m, __builtin_fmaf(k.omk, va, k.kth)), 0.0f);
 }
 
 __device__ __forceinline__ void heston_pair_px(uint32_t p, int32_t steps, const HestonPx& k,
-                                               uint32_t k0, uint32_t k1, float& xf, float& xa,
-                                               float& z1f) {
+                                               uint32_t k0, uint32_t k1, const float2* trig,
+                                               float& xf, float& xa, float& z1f) {
     float y = 0.0f, v = k.v0, ya = 0.0f, va = k.v0, lg = 0.0f, c1 = 0.0f, lg0 = 0.0f, c10 = 0.0f;
     const int32_t nb = steps / 3, rem = steps - 3 * nb;
     for (int32_t b = 0; b < nb; ++b) {
         const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
-        heston_px_pair_step(k, y, v, ya, va, u23(w.x), rev23(w.w), lg, c1);
+        const TrigOff t = trig_offsets(w);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.x), trig_at(trig, t.o[0]), lg, c1);
         if (b == 0) {
             lg0 = lg;
             c10 = c1;
         }
-        heston_px_pair_step(k, y, v, ya, va, u23(w.y), rev23(top_bytes(w.x, w.y)), lg, c1);
-        heston_px_pair_step(k, y, v, ya, va, u23(w.z), rev23(top_bytes(w.z, w.w)), lg, c1);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.y), trig_at(trig, t.o[1]), lg, c1);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.z), trig_at(trig, t.o[2]), lg, c1);
     }
     if (rem) {
         const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
-        heston_px_pair_step(k, y, v, ya, va, u23(w.x), rev23(w.w), lg, c1);
+        const TrigOff t = trig_offsets(w);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.x), trig_at(trig, t.o[0]), lg, c1);
         if (nb == 0) {
             lg0 = lg;
             c10 = c1;
         }
         if (rem > 1)
-            heston_px_pair_step(k, y, v, ya, va, u23(w.y), rev23(top_bytes(w.x, w.y)), lg, c1);
+            heston_px_pair_step(k, y, v, ya, va, u23(w.y), trig_at(trig, t.o[1]), lg, c1);
     }
     xf = y * k.ck;
     xa = ya * k.ck;
@@ -706,8 +723,8 @@ __device__ __forceinline__ HestonPx heston_px_consts(const OptionRec& rec) {
     kp.h = float(rec.c[1] / ck);
     kp.omk = float(1.0 - rec.c[3]);
     kp.kth = float(rec.c[4]);
-    kp.sgk = float(rec.c[5] * sq);
-    kp.phi = float(atan2(rec.c[7], rec.c[6]) * 0.15915494309189533577);  // phi / 2 pi
+    kp.A = float(rec.c[5] * sq * rec.c[6]);  // sigma sqrt(dt) sqrt(2 ln 2) rho
+    kp.B = float(rec.c[5] * sq * rec.c[7]);  // ... rho'
     kp.yf = float(log2(1e-12 / rec.spot) / ck);
     kp.v0 = float(rec.v0);
     kp.ck = float(ck);
@@ -727,6 +744,12 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                                                          int32_t lane_paths) {
     __shared__ int32_t s_tile;
     __shared__ double s_red[kBlock / 64][5];
+    const float2* trig = nullptr;
+    if constexpr (MODE == SIM_PHILOX_F32) {
+        __shared__ __align__(16) float2 s_trig[kTrigEntries];
+        stage_trig_table(g_trig, s_trig);
+        trig = s_trig;
+    }
     const int32_t n = *n_dev;
     const int32_t total = __builtin_amdgcn_readfirstlane(counters[1]);
     for (;;) {
@@ -766,7 +789,8 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                 }
                 R xf, xa, z1;
                 if constexpr (MODE == SIM_PHILOX_F32)
-                    heston_pair_px(uint32_t(p), steps, kp, rec.seed_lo, rec.seed_hi, xf, xa, z1);
+                    heston_pair_px(uint32_t(p), steps, kp, rec.seed_lo, rec.seed_hi, trig, xf, xa,
+                                   z1);
                 else heston_pair<R>(sp, uint32_t(p), steps, k, xf, xa, z1);
                 const R sc = no_score ? R(0) : lr_score(z1, den, inv_den, clip);
                 const R pf = payoff(spot * q_exp2(xf), strike, ot);
@@ -814,7 +838,7 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                 if (p >= paths) break;
                 R x;
                 if constexpr (MODE == SIM_PHILOX_F32)
-                    x = heston_path_px(uint32_t(p), steps, kp, rec.seed_lo, rec.seed_hi);
+                    x = heston_path_px(uint32_t(p), steps, kp, rec.seed_lo, rec.seed_hi, trig);
                 else x = heston_path<R>(ss, uint32_t(p), steps, k);
                 const double f = double(payoff(spot * q_exp2(x), strike, ot)) - shift;
                 s1 += f;
@@ -829,9 +853,9 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                 R s;
                 if constexpr (MODE == SIM_PHILOX_F32) {
                     s = mil ? gbm_path_px<true>(uint32_t(p), steps, spot, c0, c1, c2, rec.seed_lo,
-                                                rec.seed_hi)
+                                                rec.seed_hi, trig)
                             : gbm_path_px<false>(uint32_t(p), steps, spot, c0, c1, c2, rec.seed_lo,
-                                                 rec.seed_hi);
+                                                 rec.seed_hi, trig);
                 } else {
                     s = mil ? gbm_path<R, true>(ss, uint32_t(p), steps, spot, c0, c1, c2)
                             : gbm_path<R, false>(ss, uint32_t(p), steps, spot, c0, c1, c2);
@@ -1064,6 +1088,12 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
                                                          int32_t lane_groups) {
     __shared__ int32_t s_tile;
     __shared__ double s_red[kBlock / 64][5];
+    const float2* trig = nullptr;
+    if constexpr (MODE == SIM_PHILOX_F32) {
+        __shared__ __align__(16) float2 s_trig[kTrigEntries];
+        stage_trig_table(g_trig, s_trig);
+        trig = s_trig;
+    }
     const int32_t n = *n_dev;
     const int32_t total = __builtin_amdgcn_readfirstlane(counters[1]);
     for (;;) {
@@ -1094,7 +1124,7 @@ __global__ __launch_bounds__(kBlock) void k_sim_terminal(const OptionRec* __rest
         k.n = double(paths);
         const double shift = rec.shift;
         const bool anti = method_is_antithetic_terminal(method);
-        PhiloxSrc ph{rec.seed_lo, rec.seed_hi};
+        PhiloxSrc ph{rec.seed_lo, rec.seed_hi, 0u, trig};
         StreamSrc<R> ss{stream + rec.stream_off, 1, int64_t(units) + (anti ? (paths & 1) : 0)};
 
         double s1 = 0.0, s2 = 0.0, x1 = 0.0, x2 = 0.0, xy = 0.0;
@@ -1269,6 +1299,24 @@ hipError_t launch_philox_probe(hipStream_t st, uint64_t seed, uint32_t c0, uint3
 // ----------------------------------------------------------------------------
 // host launchers
 // ----------------------------------------------------------------------------
+hipError_t upload_trig_table() {
+    // (cos, sin) of 2 pi a / 4096: a quarter turn in FP64 (the C library's
+    // cos / sin of an exact multiple of 2 pi / 4096, rounded once to FP32),
+    // the other three quarters by exact symmetry, so sign and zero patterns
+    // are exact (entry 1024 is (0, 1)).
+    static float2 host[kTrigEntries];
+    constexpr int Q = kTrigEntries / 4;
+    const double step = 6.283185307179586476925286766559 / kTrigEntries;
+    for (int a = 0; a < Q; ++a) {
+        const float c = float(std::cos(step * a)), s = float(std::sin(step * a));
+        host[a] = make_float2(c, s);
+        host[a + Q] = make_float2(-s, c);
+        host[a + 2 * Q] = make_float2(-c, -s);
+        host[a + 3 * Q] = make_float2(s, -c);
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_trig), host, sizeof(host), 0, hipMemcpyHostToDevice);
+}
+
 hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_dev, int32_t n_host,
                           int32_t* n_out, const int64_t* stream_off, int32_t tg_terminal,
                           int32_t tu_stepwise, OptionRec* recs, int32_t* ntiles) {

@@ -13,15 +13,19 @@
 //
 // Six normals per block (philox_normals6): three pairs, slot s taking
 //   u1 = (2m + 1) 2^-24, m = the low 23 bits of w_s  (exact in FP32)
-// and its angle from bits that no slot's m uses:
-//   slot 0: the low 23 bits of w3;
-//   slot 1: bits 24-30 of w0 and 24-31 of w1 (15 bits);
-//   slot 2: bits 24-30 of w2 and 24-31 of w3 (15 bits).
-// The radius is capped at sqrt(48 ln 2) = 5.77 sigma.  Draw k of a terminal
-// estimator is component k%6 of block (k/6, 0, 0, 1) (cosine of slot
-// (k%6)/2 for even k, sine for odd k).  Heston paths take step j from slot
-// j%3 of block (p, j/3, 0, 0) (both components), stepwise GBM paths step j
-// from component j%2 of the pair of Heston step j/2.
+// and a 12-bit angle index a_s, theta_s = 2 pi a_s / 4096, from bits that no
+// slot's m uses:
+//   slot 0: bits 3-14 of w3;
+//   slot 1: bits 19-23 of w3 (low) and 24-30 of w0 (high);
+//   slot 2: bits 27-31 of w1 (low) and 24-30 of w2 (high);
+// (z_2s, z_2s+1) = sqrt(-2 ln u1) (cos theta_s, sin theta_s), with the
+// cosine and sine read from a 4096-entry table (trig_table_entry) that each
+// workgroup stages in LDS.  The radius is capped at sqrt(48 ln 2) = 5.77
+// sigma; 12-bit angles reproduce the trigonometric moments of a uniform angle
+// up to order 4095.  Draw k of a terminal estimator is component k%6 of block
+// (k/6, 0, 0, 1) (cosine of slot (k%6)/2 for even k, sine for odd k).  Heston
+// paths take step j from slot j%3 of block (p, j/3, 0, 0) (both components),
+// stepwise GBM paths step j from component j%2 of the pair of Heston step j/2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -102,34 +106,54 @@ struct Normal4 {
 };
 
 // Three-pair layout helpers.  u23: (2m + 1) 2^-24 from the low 23 bits m of
-// w, exact in FP32 and in (0, 1).  rev23: 1 + fraction in [1, 2) (an angle in
-// revolutions for v_sin/v_cos).  top_bytes: {0, w_hi.byte3, w_lo.byte3, 0}
-// (one v_perm_b32), the 15-bit angle fraction of slots 1 and 2.
+// w, exact in FP32 and in (0, 1).
 __device__ __forceinline__ float u23(uint32_t w) {
     return __builtin_bit_cast(float, (w & 0x007FFFFFu) | 0x3F800000u) - 0x1.fffffep-1f;
 }
-__device__ __forceinline__ float rev23(uint32_t bits) {
-    return __builtin_bit_cast(float, (bits & 0x007FFFFFu) | 0x3F800000u);
+
+constexpr int kTrigEntries = 4096;  // angle table: (cos, sin) of 2 pi a / 4096
+
+// Byte offsets into the (cos, sin) table of the three slots' angles: the
+// 12-bit index sits at bits 3-14 of a word (slot 0: w3 itself; slots 1 and 2:
+// two top bytes joined by one v_perm_b32), so one and-mask gives 8 x index.
+struct TrigOff {
+    uint32_t o[3];
+};
+__device__ __forceinline__ TrigOff trig_offsets(const U4& w) {
+    TrigOff t;
+    t.o[0] = w.w & 0x7FF8u;
+    t.o[1] = __builtin_amdgcn_perm(w.x, w.w, 0x0C0C0702u) & 0x7FF8u;  // {w3.byte2, w0.byte3}
+    t.o[2] = __builtin_amdgcn_perm(w.z, w.y, 0x0C0C0703u) & 0x7FF8u;  // {w1.byte3, w2.byte3}
+    return t;
 }
-__device__ __forceinline__ uint32_t top_bytes(uint32_t w_hi, uint32_t w_lo) {
-    return __builtin_amdgcn_perm(w_hi, w_lo, 0x0C07030Cu);
+__device__ __forceinline__ float2 trig_at(const float2* trig, uint32_t byte_off) {
+    return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(trig) + byte_off);
+}
+
+// Copies the device's angle table (global memory) into a workgroup's LDS
+// copy; every thread of the workgroup takes part, then a barrier.
+__device__ __forceinline__ void stage_trig_table(const float2* __restrict__ src, float2* dst) {
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+    float4* d4 = reinterpret_cast<float4*>(dst);
+    for (int i = threadIdx.x; i < kTrigEntries / 2; i += blockDim.x) d4[i] = s4[i];
+    __syncthreads();
 }
 
 struct Normal6 {
     float z[6];
 };
 
-__device__ __forceinline__ void box_muller23(float u1, float rev, float& z0, float& z1) {
-    const float rad = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u1) * -1.3862943611198906f);
-    z0 = rad * __builtin_amdgcn_cosf(rev);
-    z1 = rad * __builtin_amdgcn_sinf(rev);
-}
-
-__device__ __forceinline__ Normal6 philox_normals6(const U4& w) {
+__device__ __forceinline__ Normal6 philox_normals6(const U4& w, const float2* trig) {
+    const TrigOff t = trig_offsets(w);
+    const float u[3] = {u23(w.x), u23(w.y), u23(w.z)};
     Normal6 n;
-    box_muller23(u23(w.x), rev23(w.w), n.z[0], n.z[1]);
-    box_muller23(u23(w.y), rev23(top_bytes(w.x, w.y)), n.z[2], n.z[3]);
-    box_muller23(u23(w.z), rev23(top_bytes(w.z, w.w)), n.z[4], n.z[5]);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const float2 cs = trig_at(trig, t.o[s]);
+        const float rad = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u[s]) * -1.3862943611198906f);
+        n.z[2 * s] = rad * cs.x;
+        n.z[2 * s + 1] = rad * cs.y;
+    }
     return n;
 }
 

@@ -131,6 +131,35 @@ def test_box_muller_exact_mapping(orc):
     assert z0.value == pytest.approx(math.sqrt(2 * 33 * math.log(2)), rel=1e-14) and z1.value == 0.0
 
 
+def _slot_angle(w, s):
+    """12-bit angle index of slot s of a three-pair block, from the words:
+    bits 3-14 of w3 (slot 0); bits 19-23 of w3 below bits 24-30 of w0
+    (slot 1); bits 27-31 of w1 below bits 24-30 of w2 (slot 2)."""
+    if s == 0:
+        return (w[3] >> 3) & 0xFFF
+    lo, hi = (w[3] >> 19, w[0]) if s == 1 else (w[1] >> 27, w[2])
+    return (lo & 0x1F) | ((hi >> 24) & 0x7F) << 5
+
+
+def test_slot_bits_are_disjoint():
+    """The three radii (low 23 bits of w0, w1, w2) and the three 12-bit angle
+    indices use disjoint bits of the block: setting one bit at a time moves
+    at most one field."""
+    def fields(w):
+        return [w[0] & 0x7FFFFF, w[1] & 0x7FFFFF, w[2] & 0x7FFFFF] + [_slot_angle(w, s) for s in range(3)]
+    used = 0
+    for word in range(4):
+        for bit in range(32):
+            w = [0, 0, 0, 0]
+            w[word] = 1 << bit
+            moved = [i for i, f in enumerate(fields(w)) if f]
+            assert len(moved) <= 1, (word, bit, moved)
+            used += len(moved)
+    assert used == 3 * 23 + 3 * 12
+    # every angle index is reachable: all-ones words give 4095 for each slot
+    assert [_slot_angle([0xFFFFFFFF] * 4, s) for s in range(3)] == [4095] * 3
+
+
 def test_heston_stream_exact_mapping(orc):
     """Step j of path p: slot j%3 of Philox block (p, j/3, 0, 0); u1 from the
     low 23 bits of word j%3 as (2m+1) 2^-24; angle bits disjoint from them.
@@ -141,13 +170,9 @@ def test_heston_stream_exact_mapping(orc):
             w = orc.philox([p, j // 3, 0, 0], [seed & 0xFFFFFFFF, seed >> 32])
             s = j % 3
             u1 = (2.0 * (w[s] & 0x7FFFFF) + 1.0) / 2.0**24
-            if s == 0:
-                fb = w[3] & 0x7FFFFF
-            else:
-                hi, lo = (w[0], w[1]) if s == 1 else (w[2], w[3])
-                fb = ((hi >> 24) & 0x7F) << 16 | (lo >> 24) << 8
+            a = _slot_angle(w, s)
             rad = math.sqrt(-2.0 * math.log(u1))
-            want = (rad * math.cos(2 * math.pi * fb / 2.0**23), rad * math.sin(2 * math.pi * fb / 2.0**23))
+            want = (rad * math.cos(2 * math.pi * a / 4096), rad * math.sin(2 * math.pi * a / 4096))
             got = orc.heston_normals(seed, p, j)
             assert got == pytest.approx(want, rel=1e-13, abs=1e-13), (p, j)
     # the largest radius: m = 0 -> u1 = 2^-24, sqrt(48 ln 2) = 5.768
@@ -189,7 +214,7 @@ def test_gbm_stream_is_heston_pairs(orc):
     # from the Philox words for slot 0 of block 1
     w = orc.philox([5, 1, 0, 0], [seed & 0xFFFFFFFF, seed >> 32])
     u1 = (2.0 * (w[0] & 0x7FFFFF) + 1.0) / 2.0**24
-    ang = 2 * math.pi * (w[3] & 0x7FFFFF) / 2.0**23
+    ang = 2 * math.pi * ((w[3] >> 3) & 0xFFF) / 4096
     rad = math.sqrt(-2.0 * math.log(u1))
     assert orc.lib.orc_philox_gbm_normal(seed, 5, 6) == pytest.approx(rad * math.cos(ang), rel=1e-13, abs=1e-13)
     assert orc.lib.orc_philox_gbm_normal(seed, 5, 7) == pytest.approx(rad * math.sin(ang), rel=1e-13, abs=1e-13)
@@ -206,13 +231,8 @@ def test_terminal_stream_exact_mapping(orc):
         w = orc.philox([k // 6, 0, 0, 1], key)
         s = (k % 6) // 2
         u1 = (2.0 * (w[s] & 0x7FFFFF) + 1.0) / 2.0**24
-        if s == 0:
-            fb = w[3] & 0x7FFFFF
-        else:
-            hi, lo = (w[0], w[1]) if s == 1 else (w[2], w[3])
-            fb = ((hi >> 24) & 0x7F) << 16 | (lo >> 24) << 8
         rad = math.sqrt(-2.0 * math.log(u1))
-        ang = 2 * math.pi * fb / 2.0**23
+        ang = 2 * math.pi * _slot_angle(w, s) / 4096
         want = rad * (math.sin(ang) if k % 2 else math.cos(ang))
         assert orc.lib.orc_philox_terminal_normal(seed, k) == pytest.approx(want, rel=1e-13, abs=1e-13), k
     # the terminal stream (fourth counter word 1) is not the Heston stream of
