@@ -480,13 +480,32 @@ QK_EXPORT int32_t qk_sur_save(const char* path);
 QK_EXPORT int32_t qk_sur_load(const char* path);
 /* Spot check: every spot_every-th query answered by the network is also
  * priced by plain MC with spot_paths paths; a difference above
- * spot_k_se * SE sends it to the full MC fallback.  spot_every = 0 disables. */
+ * spot_k_se * SE + target/2 * spot sends it to the full MC fallback.
+ * spot_every = 0 disables the checks and the batch judge. */
 QK_EXPORT int32_t qk_sur_configure(int32_t spot_every, int32_t spot_paths, double spot_k_se);
+/* Accuracy target of the served answers, RMS |price - MC expectation| / spot
+ * (default 1e-3; in (0, 1)).  Besides the per-query tolerance above, it sets
+ * the batch judge: over the spot checks of each of 16 regions of the box
+ * (halves in log K/S, T, sigma and v0), and over all checks, the mean of
+ * ((network - MC)/S)^2 - (SE/S)^2 estimates the network's mean squared
+ * error; a region (or the whole batch) with at least 32 checks whose
+ * estimate minus 3 of its standard errors exceeds target^2 has all its
+ * network answers replaced by plain MC (source 4). */
+QK_EXPORT int32_t qk_sur_set_target(double target_rms_over_spot);
+/* The batch judge of the last pricing call with spot checks: up to max_rows
+ * rows of 4 doubles {checks, implied MSE / spot^2, its standard error,
+ * rejected (0/1)}, rows 0-15 the regions (bit 0: log K/S in the upper half
+ * of the box, bit 1: T, bit 2: sigma, bit 3: v0), row 16 all checks.
+ * Returns the number of rows written (17 at most), or a negative QK_ERR_*. */
+QK_EXPORT int32_t qk_sur_last_judge(double* out_rows, int32_t max_rows);
 /* Heston prices with the same arguments as qk_mcm_heston_monte_carlo_price_batch:
  * queries inside the box (q == 0, steps == 256) are answered by the network
- * (puts by put-call parity), the others by plain MC with their own paths,
- * steps and seed; out_source[i] is 0 network, 1 MC (outside the box),
- * 2 MC (spot check failed), 3 invalid / t <= eps (NaN / intrinsic). */
+ * (calls clamped to their no-arbitrage bounds, puts by put-call parity), the
+ * others by plain MC with their own paths, steps and seed; out_source[i] is
+ * 0 network, 1 MC (outside the box), 2 MC (spot check failed),
+ * 3 invalid / t <= eps (NaN / intrinsic), 4 MC (region rejected by the
+ * batch judge).  Every MC answer equals qk_mcm_heston_monte_carlo_price_batch's
+ * for the same query. */
 QK_EXPORT int32_t qk_sur_heston_price_batch(const double* spot, const double* strike,
         const double* t, const double* r, const double* q, const double* v0,
         const double* kappa, const double* theta, const double* sigma, const double* rho,

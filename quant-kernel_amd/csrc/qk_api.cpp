@@ -1312,6 +1312,18 @@ int32_t qk_sur_heston_price_batch_device(const double* spot, const double* strik
                             option_type, paths, steps, seed, n, out_prices, out_source);
 }
 
+int32_t qk_sur_set_target(double target_rms_over_spot) {
+    QK_SUR_RC(qk::sur_set_target(target_rms_over_spot, &err_));
+}
+
+int32_t qk_sur_last_judge(double* out_rows, int32_t max_rows) {
+    if (!check_ptrs({QK_P(out_rows)})) return QK_ERR_NULL_PTR;
+    std::string err;
+    const int32_t rc = guard(err, [&] { return qk::sur_last_judge(out_rows, max_rows, &err); });
+    if (rc < 0) set_msg(err);
+    return rc;
+}
+
 int32_t qk_sur_last_timing(double* ms_network, double* ms_monte_carlo) {
     if (!qk::sur_last_timing(ms_network, ms_monte_carlo)) {
         set_msg("no surrogate pricing call has completed yet");

@@ -39,9 +39,13 @@ struct Sur {
     int64_t t = 0;  // Adam step
     int32_t spot_every = 128, spot_paths = 8192;
     double spot_k = 3.0;
-    // pricing
-    DevBuf sX, sy, act_fb, act_spot, status, closed, sur, spot_p, spot_se, fb_p, fb_se;
+    double target = 1e-3;  // RMS |price - MC expectation| / spot
+    // pricing: pass A = spot checks + out-of-box queries, pass B = queries
+    // whose spot check or region failed
+    DevBuf sX, sy, act_a, paths_a, act_spot, act_b, status, region, closed, sur, a_p, a_se, b_p,
+        b_se, judge_part, judge_stats;
     double ms_net = -1.0, ms_mc = -1.0;
+    bool have_judge = false;
 };
 
 Sur g_sur;
@@ -418,6 +422,33 @@ void sur_configure(int32_t spot_every, int32_t spot_paths, double spot_k_se) {
     g_sur.spot_k = spot_k_se;
 }
 
+int32_t sur_set_target(double target, std::string* err) {
+    if (!(target > 0.0 && target < 1.0)) {
+        if (err) *err = "surrogate accuracy target must lie in (0, 1) (per unit spot)";
+        return QK_ERR_INVALID_INPUT;
+    }
+    g_sur.target = target;
+    return QK_OK;
+}
+
+int32_t sur_last_judge(double* rows, int32_t max_rows, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    if (!g_sur.have_judge) {
+        if (err) *err = "no surrogate pricing call with spot checks has run yet";
+        return QK_ERR_INVALID_INPUT;
+    }
+    const int32_t m = std::min(max_rows, kSurRegions + 1);
+    if (m <= 0) {
+        if (err) *err = "max_rows must be positive";
+        return QK_ERR_BAD_SIZE;
+    }
+    QK_HIP(hipMemcpy(rows, g_sur.judge_stats.p, size_t(m) * 4 * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    return m;
+}
+
 bool sur_last_timing(double* ms_network, double* ms_mc) {
     if (g_sur.ms_net < 0.0) return false;
     if (ms_network) *ms_network = g_sur.ms_net;
@@ -474,12 +505,13 @@ int32_t sur_heston_price(const BatchIn& in_arg, int32_t n, bool device, double* 
     if (rc != QK_OK) return rc;
     QK_HIP(g_sur.sX.ensure(size_t(Mpad) * kInPad * 2));
     QK_HIP(g_sur.sy.ensure(size_t(Mpad) * 4));
-    QK_HIP(g_sur.act_fb.ensure(size_t(Mpad)));
-    QK_HIP(g_sur.act_spot.ensure(size_t(Mpad)));
-    QK_HIP(g_sur.status.ensure(size_t(Mpad)));
-    for (DevBuf* b : {&g_sur.closed, &g_sur.sur, &g_sur.spot_p, &g_sur.spot_se, &g_sur.fb_p,
-                      &g_sur.fb_se})
+    for (DevBuf* b : {&g_sur.act_a, &g_sur.act_spot, &g_sur.act_b, &g_sur.status, &g_sur.region})
+        QK_HIP(b->ensure(size_t(Mpad)));
+    QK_HIP(g_sur.paths_a.ensure(size_t(Mpad) * 4));
+    for (DevBuf* b : {&g_sur.closed, &g_sur.sur, &g_sur.a_p, &g_sur.a_se, &g_sur.b_p, &g_sur.b_se})
         QK_HIP(b->ensure(size_t(Mpad) * 8));
+    QK_HIP(g_sur.judge_part.ensure(size_t(sur_judge_blocks(std::max(n, 1))) * kSurRegions * 3 * 8));
+    QK_HIP(g_sur.judge_stats.ensure(size_t(kSurRegions + 1) * 4 * 8));
     double* dprice = out_price;
     int32_t* dsrc = out_source;
     if (!device) {
@@ -488,35 +520,46 @@ int32_t sur_heston_price(const BatchIn& in_arg, int32_t n, bool device, double* 
         dprice = P<double>(d->out);
         dsrc = reinterpret_cast<int32_t*>(dprice + Mpad);
     }
+    uint8_t* act_a = P<uint8_t>(g_sur.act_a);
+    uint8_t* act_spot = P<uint8_t>(g_sur.act_spot);
+    uint8_t* act_b = P<uint8_t>(g_sur.act_b);
+    int8_t* status = P<int8_t>(g_sur.status);
+    int8_t* region = P<int8_t>(g_sur.region);
     hipEvent_t e0 = d->ev_sur[0], e1 = d->ev_sur[1], e3 = d->ev_sur[2];
     QK_HIP(hipEventRecord(e0, st));
-    // 1. features, box test, spot-check sample
-    QK_HIP(sur_prepare(st, in, n, Mpad, g_sur.spot_every, g_sur.sX.p, P<uint8_t>(g_sur.act_fb),
-                       P<uint8_t>(g_sur.act_spot), P<int8_t>(g_sur.status),
+    // 1. features, box test, spot-check sample, pass A's path counts
+    QK_HIP(sur_prepare(st, in, n, Mpad, g_sur.spot_every, g_sur.spot_paths, g_sur.sX.p, act_a,
+                       P<int32_t>(g_sur.paths_a), act_spot, act_b, status, region,
                        P<double>(g_sur.closed)));
     // 2. network: every row in one fused launch
     QK_HIP(infer(st, g_sur.sX.p, g_sur.wbf.p, P<float>(g_sur.params), P<float>(g_sur.sy), Mpad));
-    QK_HIP(sur_price(st, in, n, P<float>(g_sur.sy), P<int8_t>(g_sur.status), P<double>(g_sur.sur)));
+    QK_HIP(sur_price(st, in, n, P<float>(g_sur.sy), status, P<double>(g_sur.sur)));
     QK_HIP(hipEventRecord(e1, st));
-    // 3. spot checks (plain MC on a sample of the answered queries)
-    if (g_sur.spot_every > 0) {
-        BatchIn sp = in;
-        sp.active = P<uint8_t>(g_sur.act_spot);
-        sp.paths_override = g_sur.spot_paths;
-        rc = mc_device_locked(d, sp, n, n_dev, P<double>(g_sur.spot_p), P<double>(g_sur.spot_se), err);
-        if (rc != QK_OK) return rc;
-        QK_HIP(sur_compare(st, in, n, g_sur.spot_k, P<uint8_t>(g_sur.act_spot),
-                           P<double>(g_sur.sur), P<double>(g_sur.spot_p),
-                           P<double>(g_sur.spot_se), P<uint8_t>(g_sur.act_fb),
-                           P<int8_t>(g_sur.status)));
-    }
-    // 4. fallback MC for out-of-box queries and failed spot checks
-    BatchIn fb = in;
-    fb.active = P<uint8_t>(g_sur.act_fb);
-    rc = mc_device_locked(d, fb, n, n_dev, P<double>(g_sur.fb_p), P<double>(g_sur.fb_se), err);
+    // 3. pass A: the spot checks (spot_paths paths) and the out-of-box
+    //    queries (their own paths) in one MC pipeline
+    BatchIn pa = in;
+    pa.paths = P<int32_t>(g_sur.paths_a);
+    pa.active = act_a;
+    rc = mc_device_locked(d, pa, n, n_dev, P<double>(g_sur.a_p), P<double>(g_sur.a_se), err);
     if (rc != QK_OK) return rc;
-    QK_HIP(sur_combine(st, n, P<int8_t>(g_sur.status), P<double>(g_sur.sur), P<double>(g_sur.fb_p),
-                       P<double>(g_sur.closed), dprice, dsrc));
+    // 4. per-query checks, then the batch judge over all checks
+    if (g_sur.spot_every > 0) {
+        QK_HIP(sur_compare(st, in, n, g_sur.spot_k, 0.5 * g_sur.target, act_spot,
+                           P<double>(g_sur.sur), P<double>(g_sur.a_p), P<double>(g_sur.a_se), act_b,
+                           status));
+        QK_HIP(sur_judge(st, in, n, act_spot, region, P<double>(g_sur.sur), P<double>(g_sur.a_p),
+                         P<double>(g_sur.a_se), g_sur.target, kSurJudgeSigmas,
+                         P<double>(g_sur.judge_part), P<double>(g_sur.judge_stats), act_b, status));
+    }
+    g_sur.have_judge = g_sur.spot_every > 0;
+    // 5. pass B: plain MC for failed checks and rejected regions (their own
+    //    paths); with none, its simulate launch finds no tile
+    BatchIn pb = in;
+    pb.active = act_b;
+    rc = mc_device_locked(d, pb, n, n_dev, P<double>(g_sur.b_p), P<double>(g_sur.b_se), err);
+    if (rc != QK_OK) return rc;
+    QK_HIP(sur_combine(st, n, status, P<double>(g_sur.sur), P<double>(g_sur.a_p),
+                       P<double>(g_sur.b_p), P<double>(g_sur.closed), dprice, dsrc));
     QK_HIP(hipEventRecord(e3, st));
     if (!device) {
         QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, size_t(Mpad) * 16, hipMemcpyDeviceToHost, st));

@@ -164,6 +164,8 @@ _SIGNATURES = {
     "qk_sur_heston_price_batch": (_I, [_PD] * 10 + [_PI, _PI, _PI, _PU, _I, _PD, _PI]),
     "qk_sur_heston_price_batch_device": (_I, [ct.c_void_p] * 14 + [_I, ct.c_void_p, ct.c_void_p]),
     "qk_sur_last_timing": (_I, [_PD, _PD]),
+    "qk_sur_set_target": (_I, [_D]),
+    "qk_sur_last_judge": (_I, [_PD, _I]),
 }
 
 
@@ -362,6 +364,18 @@ class QuantKernel:
         self._check(self._lib.qk_sur_heston_price_batch_device(*args, int(n), ct.c_void_p(int(out_price_ptr)),
                                                                ct.c_void_p(int(out_source_ptr))),
                     "qk_sur_heston_price_batch_device")
+
+    def sur_set_target(self, target_rms_over_spot: float = 1e-3) -> None:
+        self._check(self._lib.qk_sur_set_target(float(target_rms_over_spot)), "qk_sur_set_target")
+
+    def sur_last_judge(self) -> np.ndarray:
+        """Batch judge of the last pricing call: rows {checks, implied MSE/S^2, SE, rejected};
+        rows 0-15 the box regions, row 16 all checks."""
+        out = np.empty((17, 4))
+        rc = int(self._lib.qk_sur_last_judge(out.ctypes.data_as(_PD), 17))
+        if rc < 0:
+            self._check(rc, "qk_sur_last_judge")
+        return out[:rc]
 
     def sur_last_timing(self) -> tuple[float, float]:
         a, b = ct.c_double(), ct.c_double()

@@ -16,6 +16,7 @@
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import math
 import sys
@@ -34,11 +35,18 @@ BOX_HI = np.array([math.log(1.3), 2.0, 0.08, 5.0, 0.16, 1.0, -0.1, 0.16])
 MASTER_TRAIN, MASTER_TEST, MASTER_QUERY = 0x5EED5000, 0x5EED6000, 0x5EED7000
 
 
+# features that the Heston model needs non-negative: T, theta, v0
+_NONNEG = np.array([False, True, False, False, True, False, False, True])
+
+
 def sample_queries(n: int, master: int, expand: float = 0.0, start: int = 0) -> dict:
     """n Heston calls with features uniform in the box (each side widened by
-    ``expand`` of its width); spot uniform in [50, 150]."""
+    ``expand`` of its width, the lower sides of T, theta and v0 no further
+    than 0, so every query is a valid Heston option); spot uniform in
+    [50, 150]."""
     lo = BOX_LO - expand * (BOX_HI - BOX_LO)
     hi = BOX_HI + expand * (BOX_HI - BOX_LO)
+    lo = np.where(_NONNEG, np.maximum(lo, 0.0), lo)
     u = np.stack([wl._uniform(master, n, c, 0.0, 1.0, start) for c in range(8)], axis=1)
     f = lo + (hi - lo) * u
     spot = wl._uniform(master, n, 8, 50.0, 150.0, start)
@@ -146,13 +154,16 @@ def main(argv=None):
         print(msg, file=sys.stderr, flush=True)
 
     qk = QuantKernel()
+    command = "python -m quantkernel_amd.surrogate " + " ".join(sys.argv[1:] if argv is None else argv)
     if a.cmd == "train":
-        rep = train(qk, a.samples, a.paths, a.epochs, a.batch, a.lr, a.lr_final, a.seed, log)
+        rep = dict(command=command)
+        rep.update(train(qk, a.samples, a.paths, a.epochs, a.batch, a.lr, a.lr_final, a.seed, log))
         qk.sur_save(a.out)
+        rep["weights_sha256"] = hashlib.sha256(Path(a.out).read_bytes()).hexdigest()
         log(f"weights written to {a.out}")
     else:
         qk.sur_load(a.weights)
-        rep = {}
+        rep = dict(command=command)
     rep["evaluation"] = evaluate(qk, a.eval, a.ref_paths, log) if a.eval > 0 else None
     if a.report:
         Path(a.report).write_text(json.dumps(rep, indent=1))
