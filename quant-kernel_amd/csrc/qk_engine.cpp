@@ -88,7 +88,7 @@ void init_device() {
     Device* d = new Device();
     d->dev = want;
     d->cus = prop.multiProcessorCount;
-    d->name = prop.name;
+    d->name = prop.name[0] ? prop.name : prop.gcnArchName;  // some runtimes leave the marketing name empty
     e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&d->ev[i]);
     for (int i = 0; e == hipSuccess && i < 3; ++i) e = hipEventCreate(&d->ev_sur[i]);
