@@ -411,3 +411,65 @@ def test_planted_mistakes_are_caught(orc):
     dn = S * np.exp(mu - vol * math.sqrt(T) * g)
     price_ok = math.exp(-r * T) * (np.maximum(up - K, 0) + np.maximum(dn - K, 0)).sum() / 50000
     assert price_ok == pytest.approx(10.533371511490282, rel=1e-12)
+
+
+def _lsmc_brute_force(orc, spot, strike, t, vol, r, q, ot, paths, steps, seed, mistake=None):
+    """Longstaff-Schwartz (longstaff_schwartz.cpp:84-125, here with the
+    {1, x, x^2, x^3}, x = S/K basis) restated with numpy least squares
+    (np.linalg.lstsq, an SVD solver) instead of the oracle's normal equations
+    and Gauss-Jordan.  Paths are exact log-GBM on the oracle's Philox draws
+    (draw j-1 of path i drives date j).  `mistake` plants an error here, on
+    the CPU, to show the comparison would see it in the oracle."""
+    dt = t / steps
+    disc = math.exp(-r * dt)
+    z = np.array([[orc.lib.orc_philox_path_normal(seed, i, j) for j in range(steps)] for i in range(paths)])
+    S = np.empty((paths, steps + 1))
+    S[:, 0] = spot
+    for j in range(1, steps + 1):
+        S[:, j] = S[:, j - 1] * np.exp((r - q - 0.5 * vol * vol) * dt + vol * math.sqrt(dt) * z[:, j - 1])
+    pay = (lambda s: np.maximum(s - strike, 0.0)) if ot == 0 else (lambda s: np.maximum(strike - s, 0.0))
+    cf = pay(S[:, steps])
+    for j in range(steps - 1, 0, -1):
+        cf = cf * (disc if mistake != "no_discount" else 1.0)
+        itm = pay(S[:, j]) > 0.0
+        if itm.sum() < 3:
+            continue
+        u = S[itm, j] / strike if mistake != "unscaled" else S[itm, j]
+        cols = [np.ones_like(u), u, u * u] + ([] if mistake == "no_cubic" else [u ** 3])
+        X = np.stack(cols, axis=1)
+        beta = np.linalg.lstsq(X, cf[itm], rcond=None)[0]
+        cont = X @ beta
+        ex = pay(S[itm, j])
+        take = ex > cont
+        idx = np.nonzero(itm)[0]
+        cf[idx[take]] = ex[take]
+    return float(np.mean(cf) * disc)
+
+
+@pytest.mark.parametrize("case", [
+    (40.0, 40.0, 1.0, 0.2, 0.06, 0.0, 1, 64, 4, 11),
+    (36.0, 40.0, 1.0, 0.4, 0.06, 0.0, 1, 96, 6, 12),
+    (100.0, 95.0, 0.5, 0.3, 0.03, 0.01, 1, 80, 5, 13),
+    (100.0, 110.0, 2.0, 0.25, 0.05, 0.02, 0, 64, 4, 14),
+])
+def test_cubic_lsmc_oracle_equals_numpy_least_squares(orc, case):
+    spot, strike, t, vol, r, q, ot, paths, steps, seed = case
+    want = _lsmc_brute_force(orc, spot, strike, t, vol, r, q, ot, paths, steps, seed)
+    got, _ = orc.px("lsmc", spot, strike, t, vol, r, q, ot, paths, steps, seed, 1)
+    # the same exercise decisions: equal up to the summation order
+    assert got == pytest.approx(want, rel=1e-12, abs=1e-12)
+
+
+def test_cubic_lsmc_brute_force_sees_planted_mistakes(orc):
+    """A dropped x^3 term or an undiscounted step changes the exercise
+    decisions, and the price moves far beyond the 1e-12 the comparison
+    allows.  An unscaled x (S instead of S/K) spans the same polynomials, so
+    in exact arithmetic it fits the same continuation values: it agrees to
+    rounding, and only the conditioning of the normal equations changes."""
+    case = (36.0, 40.0, 1.0, 0.4, 0.06, 0.0, 1, 96, 6, 12)
+    got, _ = orc.px("lsmc", *case, 1)
+    for m in ("no_cubic", "no_discount"):
+        bad = _lsmc_brute_force(orc, *case, mistake=m)
+        assert abs(bad - got) > 1e-6 * got, m
+    same = _lsmc_brute_force(orc, *case, mistake="unscaled")
+    assert same == pytest.approx(got, rel=1e-9)
