@@ -78,6 +78,62 @@ def test_heston_16384_x_65536_x_256_strike_identity(qk):
     assert err.max() < 1e-5
 
 
+def test_heston_16384_x_65536_x_256_martingale_and_cf(qk):
+    """The full C1 batch through checks that depend on S_T (the strike
+    identity above cancels it path by path):
+
+    - martingale: a call struck at 1e-6 S is in the money on every path, so
+      its price is e^{-rT}(mean S_T - K); the log-Euler scheme keeps
+      E[S_T] = S e^{(r-q)T} exactly (the radius cap and the 1e-12 floor aside),
+      so z = (price - S e^{-qT} + K e^{-rT}) / SE is N(0, 1) per option and the
+      fraction within 3 SE is held to its binomial bound;
+    - the characteristic-function prices (FP64, this library; the reference's
+      heston.cpp) of the real calls: MC - CF is the full-truncation Euler bias
+      of the reference's own scheme plus noise.  It is positive on average,
+      about 7e-4 of spot at 256 steps, and shrinks with the step count: the
+      same options at 512 steps sit closer to CF.
+    """
+    d = wl.config1()
+    n = d["spot"].shape[0]
+    assert n == 16384 and int(d["paths"][0]) == 65536 and int(d["steps"][0]) == 256
+    k0 = 1e-6 * d["spot"]
+    a = [k0 if k == "strike" else d[k] for k in HESTON]
+    fwd, se = qk.heston_monte_carlo_price_batch(*a, d["option_type"], d["paths"], d["steps"], d["seed"],
+                                                return_stderr=True)
+    want = d["spot"] * np.exp(-d["q"] * d["t"]) - k0 * np.exp(-d["r"] * d["t"])
+    z = (fwd - want) / se
+    p3 = math.erf(3 / math.sqrt(2))
+    bound = p3 - 3.0 * math.sqrt(p3 * (1 - p3) / n)
+    within = float(np.mean(np.abs(z) < 3.0))
+    print(f"martingale: mean z {z.mean():.4f}, rms {math.sqrt(np.mean(z * z)):.4f}, within 3 SE {within:.5f} "
+          f"(bound {bound:.5f}), median SE/S {np.median(se / d['spot']):.2e}")
+    assert abs(z.mean()) < 4.0 / math.sqrt(n)
+    assert 0.95 < math.sqrt(np.mean(z * z)) < 1.05
+    assert within >= bound
+    # the real calls against the characteristic function
+    price, pse = qk.heston_monte_carlo_price_batch(*[d[k] for k in HESTON], d["option_type"], d["paths"],
+                                                   d["steps"], d["seed"], return_stderr=True)
+    cf = qk.heston_price_cf_batch(*[d[k] for k in HESTON], d["option_type"], np.full(n, 1024, np.int32),
+                                  np.full(n, 120.0))
+    bias = (price - cf) / d["spot"]
+    print(f"MC - CF: mean {bias.mean():.3e} S (SE of the mean {np.std(bias) / math.sqrt(n):.1e}), "
+          f"p1 {np.quantile(bias, 0.01):.2e} p99 {np.quantile(bias, 0.99):.2e} max |.| {np.abs(bias).max():.2e}")
+    assert np.all(np.isfinite(cf))
+    assert 3e-4 < bias.mean() < 1.2e-3
+    # the Euler bias depends on the parameters (p99 about 1e-2 of spot where
+    # the Feller condition fails hardest): per option only a gross-error
+    # guard, 5 SE plus 5e-2 of spot
+    assert np.all(np.abs(price - cf) <= 5.0 * pse + 5e-2 * d["spot"])
+    # halving the time step moves the options toward CF
+    idx = np.arange(0, n, 8)
+    sub = {k: v[idx] for k, v in d.items()}
+    p512 = qk.heston_monte_carlo_price_batch(*[sub[k] for k in HESTON], sub["option_type"], sub["paths"],
+                                             np.full(len(idx), 512, np.int32), sub["seed"])
+    b256, b512 = bias[idx].mean(), ((p512 - cf[idx]) / sub["spot"]).mean()
+    print(f"Euler bias on {len(idx)} options: 256 steps {b256:.3e} S, 512 steps {b512:.3e} S")
+    assert b512 < b256 - 5e-5
+
+
 def test_mixed_50000_uneven_batch(qk, orc):
     d = wl.config2()
     n = d["spot"].shape[0]
