@@ -522,8 +522,8 @@ __device__ __forceinline__ R heston_path(const Src& src, uint32_t p, int32_t ste
 // Heston stream: Philox block (p, j/3, 0, 0) drives steps 3(j/3) .. 3(j/3)+2
 // of path p.  Slot s of the block (words w0..w3):
 //   u1 = (2m + 1) 2^-24, m = low 23 bits of w_s (exact in FP32, in (0,1));
-//   (cos, sin) of the 12-bit angle index of slot s from the LDS table
-//   (trig_offsets: bits disjoint from all three m).
+//   (cos, A cos + B sin) of the 12-bit angle index of slot s from the LDS
+//   table (trig_offsets: bits disjoint from all three m).
 // One block per three steps instead of two saves a third of the Philox work;
 // the table replaces the two cosines of a step, whose transcendental issue
 // cost three FMAs each; the radius is capped at sqrt(48 ln 2) = 5.77 sigma.
@@ -538,13 +538,43 @@ __device__ __forceinline__ float floor_max(float x, float f) {
 }
 
 // sqrt(v) r' = sqrt(-v log2 u1): one square root per step instead of two.
+// cm = (cos(theta), A cos(theta) + B sin(theta)) from the Heston table.
 __device__ __forceinline__ void heston_px_step(const HestonPx& k, float& y, float& v, float u1,
-                                               float2 cs) {
+                                               float2 cm) {
     const float lg = __builtin_amdgcn_logf(u1);  // < 0
     const float t = __builtin_amdgcn_sqrtf(-(v * lg));
-    y = floor_max(__builtin_fmaf(t, cs.x, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
-    const float m = __builtin_fmaf(k.B, cs.y, k.A * cs.x);
-    v = __builtin_fmaxf(__builtin_fmaf(t, m, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
+    y = floor_max(__builtin_fmaf(t, cm.x, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
+    v = __builtin_fmaxf(__builtin_fmaf(t, cm.y, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
+}
+
+// The stepwise kernel's LDS angle table holds (cos, sin) for GBM tiles and,
+// for Heston tiles, (cos, fma(B, sin, A cos)) with the tile option's A and
+// B: the variance step's correlated term costs one lookup instead of a
+// multiply and an FMA per step.  The entry is rounded exactly as the
+// per-step expression was, so prices are unchanged bitwise.  Rebuilt from
+// the device table (global memory) only when a tile needs other contents
+// than the last one; every thread of the workgroup calls it with the same
+// arguments.
+struct TrigTableState {
+    int32_t kind;  // 0: (cos, sin); 1: Heston (cos, m) for (A, B)
+    float A, B;
+};
+
+__device__ __forceinline__ void retable(float2* s_trig, TrigTableState* st, bool heston,
+                                        float A, float B) {
+    const int32_t want = heston ? 1 : 0;
+    if (st->kind == want && (!heston || (st->A == A && st->B == B))) return;
+    __syncthreads();  // every thread has read st and finished with the old contents
+    for (int i = threadIdx.x; i < kTrigEntries; i += kBlock) {
+        const float2 c = g_trig[i];
+        s_trig[i] = heston ? make_float2(c.x, __builtin_fmaf(B, c.y, A * c.x)) : c;
+    }
+    if (threadIdx.x == 0) {
+        st->kind = want;
+        st->A = A;
+        st->B = B;
+    }
+    __syncthreads();
 }
 
 __device__ __forceinline__ float heston_path_px(uint32_t p, int32_t steps, const HestonPx& k,
@@ -650,14 +680,15 @@ __device__ __forceinline__ void heston_pair(const Src& src, uint32_t p, int32_t 
 }
 
 // Production form of heston_pair: one Box-Muller evaluation drives both
-// paths (negating both normals is negating the radius).  lg and c1 return
-// log2 u1 and cos(theta) of the step (z1 = sqrt(2 ln 2) sqrt(-lg) c1).
+// paths (negating both normals is negating the radius).  cs is a Heston
+// table entry (cos, A cos + B sin).  lg and c1 return log2 u1 and
+// cos(theta) of the step (z1 = sqrt(2 ln 2) sqrt(-lg) c1).
 __device__ __forceinline__ void heston_px_pair_step(const HestonPx& k, float& y, float& v,
                                                     float& ya, float& va, float u1, float2 cs,
                                                     float& lg, float& c1) {
     lg = __builtin_amdgcn_logf(u1);
     c1 = cs.x;
-    const float m = __builtin_fmaf(k.B, cs.y, k.A * cs.x);
+    const float m = cs.y;  // A cos + B sin from the Heston table
     const float t = __builtin_amdgcn_sqrtf(-(v * lg));
     y = floor_max(__builtin_fmaf(t, c1, y + __builtin_fmaf(-k.h, v, k.a)), k.yf);
     v = __builtin_fmaxf(__builtin_fmaf(t, m, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
@@ -744,10 +775,12 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                                                          int32_t lane_paths) {
     __shared__ int32_t s_tile;
     __shared__ double s_red[kBlock / 64][5];
+    __shared__ __align__(16) float2 s_trig[MODE == SIM_PHILOX_F32 ? kTrigEntries : 1];
+    __shared__ TrigTableState s_tts;
     const float2* trig = nullptr;
     if constexpr (MODE == SIM_PHILOX_F32) {
-        __shared__ __align__(16) float2 s_trig[kTrigEntries];
-        stage_trig_table(g_trig, s_trig);
+        if (threadIdx.x == 0) s_tts = TrigTableState{0, 0.0f, 0.0f};
+        stage_trig_table(g_trig, s_trig);  // (cos, sin); ends with a barrier
         trig = s_trig;
     }
     const int32_t n = *n_dev;
@@ -759,6 +792,12 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
         const OptionRec& rec = recs[opt];
         const int32_t method = rec.method, steps = rec.steps, paths = rec.paths, ot = rec.ot;
         const int64_t first = int64_t(tile - rec.tile_first) * kBlock * lane_paths;
+        const bool heston_tile = method == M_HESTON || method == M_HESTON_LR;
+        HestonPx kp{};
+        if constexpr (MODE == SIM_PHILOX_F32) {
+            if (heston_tile) kp = heston_px_consts(rec);
+            retable(s_trig, &s_tts, heston_tile, kp.A, kp.B);
+        }
         const R spot = R(rec.spot), strike = R(rec.strike);
         const double shift = rec.shift;
 
@@ -771,8 +810,6 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
             // path (Philox path index / stream unit = pairs) on lane 0 of the
             // option's last tile
             const HestonC<R> k = heston_consts<R>(rec);
-            HestonPx kp;
-            if constexpr (MODE == SIM_PHILOX_F32) kp = heston_px_consts(rec);
             const int32_t pairs = rec.units;
             StreamSrc<R> sp{stream + rec.stream_off, int64_t(pairs) + (paths & 1),
                             int64_t(steps) * 2};
@@ -831,8 +868,6 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
             }
         } else if (method == M_HESTON) {
             const HestonC<R> k = heston_consts<R>(rec);
-            HestonPx kp;
-            if constexpr (MODE == SIM_PHILOX_F32) kp = heston_px_consts(rec);
             for (int32_t j = 0; j < lane_paths; ++j) {
                 const int64_t p = first + threadIdx.x + int64_t(j) * kBlock;
                 if (p >= paths) break;
