@@ -510,6 +510,26 @@ def c2_cpu_baseline(d, budget_s=10.0, budget_1t_s=5.0):
                            "sample": f"every {n / k1:.0f}th option ({k1}), wall time {dt1:.2f} s"}}
 
 
+def copy_bandwidth(mib=2048, reps=5):
+    """What a plain streaming kernel moves on this GPU: the runtime's
+    device-to-device copy of mib MiB, read + written bytes per second (best
+    of reps synchronous copies after one warm-up)."""
+    n = mib << 18  # fp32 elements
+    x = devmem.DevArray(n=n, dtype=np.float32)
+    y = devmem.DevArray(n=n, dtype=np.float32)
+    x.fill_bytes(0)
+    y.copy_from(x)
+    devmem.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        y.copy_from(x)
+        devmem.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    del x, y
+    return 2.0 * n * 4 / best / 1e9
+
+
 def bench_c3(args, qk, qka, dev, name, cus):
     from quantkernel_amd import workloads as wl
     n = args.options or 1024
@@ -533,6 +553,7 @@ def bench_c3(args, qk, qka, dev, name, cus):
     paths, dates = 100000, 50
     byts = float(n) * paths * (4 * dates + 16 * (dates - 2) + 8)
     ach = byts / (dev_ms * 1e-3) / 1e9
+    copy_gbps = copy_bandwidth()
     res = dict(value=n / (ms * 1e-3), ms_per_step=ms, dtype="fp32",
                config={"workload": f"C3: {n} American puts, Longstaff-Schwartz, {paths} paths x {dates} dates, basis "
                                    "{1,x,x^2,x^3}, x=S/K (Philox, FP32 paths, FP64 normal equations); host entry",
@@ -542,7 +563,10 @@ def bench_c3(args, qk, qka, dev, name, cus):
                          "traffic": None,
                          "kernel": "LSMC pipeline: k_lsmc_forward, 49 x (k_lsmc_solve, k_lsmc_sweep), k_lsmc_final",
                          "kernel_ms": dev_ms,
-                         "bytes_per_launch": byts})
+                         "bytes_per_launch": byts,
+                         "plain_copy_gbps": copy_gbps,
+                         "plain_copy": "hipMemcpy device-to-device of 2 GiB on the same box, "
+                                       "read + written bytes per second (best of 5)"})
     if not args.no_cpu:
         kind, fn = ref_fn("qk_mcm_longstaff_schwartz_price_batch")
         if fn:
