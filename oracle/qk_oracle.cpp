@@ -138,8 +138,9 @@ double philox_path_normal(uint64_t seed, uint32_t p, uint32_t d) {
     return z[d & 3u];
 }
 
-// Three-pair layout of one Philox block (counter in[], key = seed): slot s
-// takes u1 = (2m + 1) 2^-24 with m the low 23 bits of word s, and a 12-bit
+// Three-pair layout of one Philox block (the terminal stream; counter in[],
+// key = seed): slot s takes u1 = (2m + 1) 2^-24 with m the low 23 bits of
+// word s, and a 12-bit
 // angle index a: bits 3-14 of w3 (slot 0); bits 19-23 of w3 below bits 24-30
 // of w0 (slot 1); bits 27-31 of w1 below bits 24-30 of w2 (slot 2).
 // (z1, z2) = sqrt(-2 ln u1) (cos 2 pi a / 4096, sin 2 pi a / 4096) in FP64
@@ -162,17 +163,34 @@ void philox_slot_pair(const uint32_t in[4], uint64_t seed, uint32_t s, double* z
     *z2 = rad * std::sin(ang);
 }
 
-// Heston (production) stream: step j of path p uses slot j%3 of block
-// (p, j/3, 0, 0).
+// Four-pair layout of one Philox block (the stepwise streams): slot s is word
+// s alone, u1 = (2m + 1) 2^-24 with m its low 23 bits and a 9-bit angle
+// index a = its top 9 bits; (z1, z2) = sqrt(-2 ln u1) (cos 2 pi a / 512,
+// sin 2 pi a / 512) in FP64 (the GPU reads FP32 roundings of the same
+// cosine and sine from a table).
+void philox_quad_pair(const uint32_t in[4], uint64_t seed, uint32_t s, double* z1, double* z2) {
+    const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    uint32_t w[4];
+    philox10(in, key, w);
+    const double u1 = (2.0 * double(w[s] & 0x007FFFFFu) + 1.0) * 0x1p-24;
+    const double f = double(w[s] >> 23) * (1.0 / 512.0);
+    const double rad = std::sqrt(-2.0 * std::log(u1));
+    const double ang = 6.283185307179586476925286766559 * f;
+    *z1 = rad * std::cos(ang);
+    *z2 = rad * std::sin(ang);
+}
+
+// Heston (production) stream: step j of path p uses slot j%4 of block
+// (p, j/4, 0, 0) in the four-pair layout.
 void philox_heston_normals(uint64_t seed, uint32_t p, uint32_t j, double* z1, double* z2) {
-    const uint32_t in[4] = {p, j / 3u, 0u, 0u};
-    philox_slot_pair(in, seed, j % 3u, z1, z2);
+    const uint32_t in[4] = {p, j / 4u, 0u, 0u};
+    philox_quad_pair(in, seed, j % 4u, z1, z2);
 }
 
 // GBM / local-vol stepwise (production) stream: step j of path p takes
-// component j%2 of the Heston pair of step j/2, i.e. slot (j%6)/2 of block
-// (p, j/6, 0, 0), the cosine for even j and the sine for odd j: six normals
-// per Philox block.
+// component j%2 of the Heston pair of step j/2, i.e. slot (j%8)/2 of block
+// (p, j/8, 0, 0), the cosine for even j and the sine for odd j: eight
+// normals per Philox block.
 double philox_gbm_normal(uint64_t seed, uint32_t p, uint32_t j) {
     double z1, z2;
     philox_heston_normals(seed, p, j >> 1, &z1, &z2);
@@ -567,7 +585,7 @@ ORC_EXPORT double orc_px_terminal(double spot, double strike, double t, double v
 }
 
 // Stepwise 1-D models: model 0 Euler, 1 Milstein, 3 constant local vol, on the
-// six-normals-per-block stream of philox_gbm_normal.
+// eight-normals-per-block stream of philox_gbm_normal.
 ORC_EXPORT double orc_px_stepwise(int32_t model, double spot, double strike, double t, double vol,
                                   double r, double q, int32_t ot, int32_t paths, int32_t steps,
                                   uint64_t seed, double* se) {

@@ -78,7 +78,7 @@ struct Draws {
 struct PhiloxSrc {
     uint32_t k0, k1;
     uint32_t tag = 0;               // fourth counter word of path blocks (the MLMC level)
-    const float2* trig = nullptr;   // LDS angle table of the three-pair layout
+    const float2* trig = nullptr;   // LDS angle table of the three-pair (terminal) layout
     __device__ __forceinline__ Quad<float> path_block(uint32_t p, uint32_t b, int32_t) const {
         const Normal4 n = philox_normals(p, b, tag, k0, k1);
         return Quad<float>{{n.z0, n.z1, n.z2, n.z3}};
@@ -519,14 +519,18 @@ __device__ __forceinline__ R heston_path(const Src& src, uint32_t p, int32_t ste
 // B sin(theta))), with r' = sqrt(-log2 u1), A = sigma sqrt(dt) sqrt(2 ln 2) rho
 // and B the same with rho' = sqrt(1 - rho^2).
 //
-// Heston stream: Philox block (p, j/3, 0, 0) drives steps 3(j/3) .. 3(j/3)+2
-// of path p.  Slot s of the block (words w0..w3):
+// Heston stream (four-pair layout): Philox block (p, j/4, 0, 0) drives
+// steps 4(j/4) .. 4(j/4)+3 of path p; step j takes word s = j%4 alone:
 //   u1 = (2m + 1) 2^-24, m = low 23 bits of w_s (exact in FP32, in (0,1));
-//   (cos, A cos + B sin) of the 12-bit angle index of slot s from the LDS
-//   table (trig_offsets: bits disjoint from all three m).
-// One block per three steps instead of two saves a third of the Philox work;
-// the table replaces the two cosines of a step, whose transcendental issue
-// cost three FMAs each; the radius is capped at sqrt(48 ln 2) = 5.77 sigma.
+//   theta = 2 pi a / 512, a = top 9 bits of w_s, read as (cos, A cos + B sin)
+//   from the LDS table.
+// The radius is capped at sqrt(48 ln 2) = 5.77 sigma.  The angle takes 512
+// equally spaced values: E[cos^n theta] and E[sin^n theta] equal their
+// values under a uniform angle for every n < 512, so every moment of
+// (z1, z2) below order 512 is that of the Gaussian pair the 23-bit radius
+// gives.  One block per four steps (instead of two with four full
+// Box-Muller normals) quarters the Philox work per step; the table replaces
+// the two cosines of a step, whose transcendental issue cost three FMAs each.
 struct HestonPx {
     float a, h, omk, kth, A, B, yf, v0, ck;
 };
@@ -547,14 +551,14 @@ __device__ __forceinline__ void heston_px_step(const HestonPx& k, float& y, floa
     v = __builtin_fmaxf(__builtin_fmaf(t, cm.y, __builtin_fmaf(k.omk, v, k.kth)), 0.0f);
 }
 
-// The stepwise kernel's LDS angle table holds (cos, sin) for GBM tiles and,
-// for Heston tiles, (cos, fma(B, sin, A cos)) with the tile option's A and
-// B: the variance step's correlated term costs one lookup instead of a
-// multiply and an FMA per step.  The entry is rounded exactly as the
-// per-step expression was, so prices are unchanged bitwise.  Rebuilt from
-// the device table (global memory) only when a tile needs other contents
-// than the last one; every thread of the workgroup calls it with the same
-// arguments.
+// The stepwise kernel's LDS angle table (512 entries, the angles of the
+// four-pair layout) holds (cos, sin) for GBM tiles and, for Heston tiles,
+// (cos, fma(B, sin, A cos)) with the tile option's A and B: the variance
+// step's correlated term costs one lookup instead of a multiply and an FMA
+// per step (the entry is rounded exactly as that per-step expression would
+// be).  Rebuilt from the device table (global memory) only when a tile
+// needs other contents than the last one; every thread of the workgroup
+// calls it with the same arguments.
 struct TrigTableState {
     int32_t kind;  // 0: (cos, sin); 1: Heston (cos, m) for (A, B)
     float A, B;
@@ -565,8 +569,8 @@ __device__ __forceinline__ void retable(float2* s_trig, TrigTableState* st, bool
     const int32_t want = heston ? 1 : 0;
     if (st->kind == want && (!heston || (st->A == A && st->B == B))) return;
     __syncthreads();  // every thread has read st and finished with the old contents
-    for (int i = threadIdx.x; i < kTrigEntries; i += kBlock) {
-        const float2 c = g_trig[i];
+    for (int i = threadIdx.x; i < kQuadAngles; i += kBlock) {
+        const float2 c = g_trig[i * (kTrigEntries / kQuadAngles)];  // angle 2 pi i / 512
         s_trig[i] = heston ? make_float2(c.x, __builtin_fmaf(B, c.y, A * c.x)) : c;
     }
     if (threadIdx.x == 0) {
@@ -581,28 +585,28 @@ __device__ __forceinline__ float heston_path_px(uint32_t p, int32_t steps, const
                                                 uint32_t k0, uint32_t k1,
                                                 const float2* trig) {
     float y = 0.0f, v = k.v0;
-    const int32_t nb = steps / 3, rem = steps - 3 * nb;
+    const int32_t nb = steps / 4, rem = steps - 4 * nb;
     for (int32_t b = 0; b < nb; ++b) {
         const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
-        const TrigOff t = trig_offsets(w);
-        const float2 c0 = trig_at(trig, t.o[0]), c1 = trig_at(trig, t.o[1]),
-                     c2 = trig_at(trig, t.o[2]);
+        const float2 c0 = trig_at(trig, quad_offset(w.x)), c1 = trig_at(trig, quad_offset(w.y)),
+                     c2 = trig_at(trig, quad_offset(w.z)), c3 = trig_at(trig, quad_offset(w.w));
         heston_px_step(k, y, v, u23(w.x), c0);
         heston_px_step(k, y, v, u23(w.y), c1);
         heston_px_step(k, y, v, u23(w.z), c2);
+        heston_px_step(k, y, v, u23(w.w), c3);
     }
     if (rem) {
         const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
-        const TrigOff t = trig_offsets(w);
-        heston_px_step(k, y, v, u23(w.x), trig_at(trig, t.o[0]));
-        if (rem > 1) heston_px_step(k, y, v, u23(w.y), trig_at(trig, t.o[1]));
+        heston_px_step(k, y, v, u23(w.x), trig_at(trig, quad_offset(w.x)));
+        if (rem > 1) heston_px_step(k, y, v, u23(w.y), trig_at(trig, quad_offset(w.y)));
+        if (rem > 2) heston_px_step(k, y, v, u23(w.z), trig_at(trig, quad_offset(w.z)));
     }
     return y * k.ck;  // log2(S_T / spot)
 }
 
 // Production GBM Euler / Milstein / local-vol path: the scheme of gbm_path on
 // the Heston stream's pairs, step j taking the cosine (even j) or sine (odd
-// j) of the pair of Heston step j/2, so one Philox block drives six steps.
+// j) of the pair of Heston step j/2, so one Philox block drives eight steps.
 // m = -2 ln 2 c1^2 for Euler and local vol folds the volatility into the
 // radius (c1 z = sqrt(m log2 u1) cos theta); Milstein keeps z (m = -2 ln 2).
 template <bool MIL>
@@ -631,24 +635,25 @@ __device__ __forceinline__ float gbm_path_px(uint32_t p, int32_t steps, float s,
                                              float c2, uint32_t k0, uint32_t k1,
                                              const float2* trig) {
     const float m = MIL ? -1.3862943611198906f : -1.3862943611198906f * c1 * c1;
-    const int32_t nb = steps / 6, rem = steps - 6 * nb;
+    const int32_t nb = steps / 8, rem = steps - 8 * nb;
     for (int32_t b = 0; b < nb; ++b) {
         const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
-        const TrigOff t = trig_offsets(w);
-        const float2 a0 = trig_at(trig, t.o[0]), a1 = trig_at(trig, t.o[1]),
-                     a2 = trig_at(trig, t.o[2]);
+        const float2 a0 = trig_at(trig, quad_offset(w.x)), a1 = trig_at(trig, quad_offset(w.y)),
+                     a2 = trig_at(trig, quad_offset(w.z)), a3 = trig_at(trig, quad_offset(w.w));
         gbm_px_pair<MIL>(s, u23(w.x), a0, c0, c1, c2, m, true);
         gbm_px_pair<MIL>(s, u23(w.y), a1, c0, c1, c2, m, true);
         gbm_px_pair<MIL>(s, u23(w.z), a2, c0, c1, c2, m, true);
+        gbm_px_pair<MIL>(s, u23(w.w), a3, c0, c1, c2, m, true);
     }
     if (rem) {
         const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
-        const TrigOff t = trig_offsets(w);
-        gbm_px_pair<MIL>(s, u23(w.x), trig_at(trig, t.o[0]), c0, c1, c2, m, rem > 1);
+        gbm_px_pair<MIL>(s, u23(w.x), trig_at(trig, quad_offset(w.x)), c0, c1, c2, m, rem > 1);
         if (rem > 2)
-            gbm_px_pair<MIL>(s, u23(w.y), trig_at(trig, t.o[1]), c0, c1, c2, m, rem > 3);
+            gbm_px_pair<MIL>(s, u23(w.y), trig_at(trig, quad_offset(w.y)), c0, c1, c2, m, rem > 3);
         if (rem > 4)
-            gbm_px_pair<MIL>(s, u23(w.z), trig_at(trig, t.o[2]), c0, c1, c2, m, false);
+            gbm_px_pair<MIL>(s, u23(w.z), trig_at(trig, quad_offset(w.z)), c0, c1, c2, m, rem > 5);
+        if (rem > 6)
+            gbm_px_pair<MIL>(s, u23(w.w), trig_at(trig, quad_offset(w.w)), c0, c1, c2, m, false);
     }
     return s;
 }
@@ -701,28 +706,29 @@ __device__ __forceinline__ void heston_pair_px(uint32_t p, int32_t steps, const 
                                                uint32_t k0, uint32_t k1, const float2* trig,
                                                float& xf, float& xa, float& z1f) {
     float y = 0.0f, v = k.v0, ya = 0.0f, va = k.v0, lg = 0.0f, c1 = 0.0f, lg0 = 0.0f, c10 = 0.0f;
-    const int32_t nb = steps / 3, rem = steps - 3 * nb;
+    const int32_t nb = steps / 4, rem = steps - 4 * nb;
     for (int32_t b = 0; b < nb; ++b) {
         const U4 w = philox4x32_10(p, uint32_t(b), 0u, k0, k1);
-        const TrigOff t = trig_offsets(w);
-        heston_px_pair_step(k, y, v, ya, va, u23(w.x), trig_at(trig, t.o[0]), lg, c1);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.x), trig_at(trig, quad_offset(w.x)), lg, c1);
         if (b == 0) {
             lg0 = lg;
             c10 = c1;
         }
-        heston_px_pair_step(k, y, v, ya, va, u23(w.y), trig_at(trig, t.o[1]), lg, c1);
-        heston_px_pair_step(k, y, v, ya, va, u23(w.z), trig_at(trig, t.o[2]), lg, c1);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.y), trig_at(trig, quad_offset(w.y)), lg, c1);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.z), trig_at(trig, quad_offset(w.z)), lg, c1);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.w), trig_at(trig, quad_offset(w.w)), lg, c1);
     }
     if (rem) {
         const U4 w = philox4x32_10(p, uint32_t(nb), 0u, k0, k1);
-        const TrigOff t = trig_offsets(w);
-        heston_px_pair_step(k, y, v, ya, va, u23(w.x), trig_at(trig, t.o[0]), lg, c1);
+        heston_px_pair_step(k, y, v, ya, va, u23(w.x), trig_at(trig, quad_offset(w.x)), lg, c1);
         if (nb == 0) {
             lg0 = lg;
             c10 = c1;
         }
         if (rem > 1)
-            heston_px_pair_step(k, y, v, ya, va, u23(w.y), trig_at(trig, t.o[1]), lg, c1);
+            heston_px_pair_step(k, y, v, ya, va, u23(w.y), trig_at(trig, quad_offset(w.y)), lg, c1);
+        if (rem > 2)
+            heston_px_pair_step(k, y, v, ya, va, u23(w.z), trig_at(trig, quad_offset(w.z)), lg, c1);
     }
     xf = y * k.ck;
     xa = ya * k.ck;
@@ -775,12 +781,12 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                                                          int32_t lane_paths) {
     __shared__ int32_t s_tile;
     __shared__ double s_red[kBlock / 64][5];
-    __shared__ __align__(16) float2 s_trig[MODE == SIM_PHILOX_F32 ? kTrigEntries : 1];
+    __shared__ __align__(16) float2 s_trig[MODE == SIM_PHILOX_F32 ? kQuadAngles : 1];
     __shared__ TrigTableState s_tts;
     const float2* trig = nullptr;
     if constexpr (MODE == SIM_PHILOX_F32) {
-        if (threadIdx.x == 0) s_tts = TrigTableState{0, 0.0f, 0.0f};
-        stage_trig_table(g_trig, s_trig);  // (cos, sin); ends with a barrier
+        if (threadIdx.x == 0) s_tts = TrigTableState{-1, 0.0f, 0.0f};  // filled by the first tile
+        __syncthreads();
         trig = s_trig;
     }
     const int32_t n = *n_dev;

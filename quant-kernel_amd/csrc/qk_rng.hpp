@@ -139,6 +139,13 @@ __device__ __forceinline__ void stage_trig_table(const float2* __restrict__ src,
     __syncthreads();
 }
 
+// Four-pair layout of the stepwise (path) streams: slot s of a block is
+// word s alone, u1 = (2m + 1) 2^-24 from its low 23 bits m and a 9-bit angle
+// index from its top 9 bits, looked up in a 512-entry table (entry a = entry
+// 8a of the three-pair table).  quad_offset gives the byte offset (8 x index).
+constexpr int kQuadAngles = 512;
+__device__ __forceinline__ uint32_t quad_offset(uint32_t w) { return (w >> 20) & 0xFF8u; }
+
 struct Normal6 {
     float z[6];
 };

@@ -182,7 +182,7 @@ def test_philox_matches_fp64_oracle(qk, orc):
     # Heston LR delta: FP32 pairs driven by one Box-Muller evaluation
     h = wl.config1(8, paths=2001, steps=33)
     h["option_type"] = (np.arange(8) % 2).astype(np.int32)
-    h["steps"] = np.array([33, 34, 35, 1, 2, 3, 4, 64], np.int32)  # every remainder mod 3
+    h["steps"] = np.array([33, 34, 35, 1, 2, 3, 4, 64], np.int32)  # every remainder mod 4
     clip = np.full(8, 6.0)
     got, se = qk.heston_lr_delta_batch(*[h[k] for k in HST], h["option_type"], h["paths"], h["steps"],
                                        h["seed"], clip, return_stderr=True)

@@ -160,23 +160,57 @@ def test_slot_bits_are_disjoint():
     assert [_slot_angle([0xFFFFFFFF] * 4, s) for s in range(3)] == [4095] * 3
 
 
+def test_quad_slot_bits():
+    """Four-pair layout: slot s is word s alone, the radius from its low 23
+    bits and the 9-bit angle index from its top 9 bits; the two fields of a
+    word share no bit and together use all 32, and every angle index is
+    reachable."""
+    for bit in range(32):
+        w = 1 << bit
+        m, a = w & 0x7FFFFF, w >> 23
+        assert (m != 0) != (a != 0), bit
+    assert 0xFFFFFFFF >> 23 == 511
+
+
 def test_heston_stream_exact_mapping(orc):
-    """Step j of path p: slot j%3 of Philox block (p, j/3, 0, 0); u1 from the
-    low 23 bits of word j%3 as (2m+1) 2^-24; angle bits disjoint from them.
-    Restated here from the words with numpy and compared for many draws."""
+    """Step j of path p: word j%4 of Philox block (p, j/4, 0, 0); u1 from its
+    low 23 bits as (2m+1) 2^-24, angle 2 pi a / 512 from its top 9 bits.
+    Restated here from the words and compared for many draws."""
     seed = 0x1234_5678_9ABC_DEF0
     for p in (0, 1, 77, 2**31 + 5):
-        for j in range(9):
-            w = orc.philox([p, j // 3, 0, 0], [seed & 0xFFFFFFFF, seed >> 32])
-            s = j % 3
+        for j in range(13):
+            w = orc.philox([p, j // 4, 0, 0], [seed & 0xFFFFFFFF, seed >> 32])
+            s = j % 4
             u1 = (2.0 * (w[s] & 0x7FFFFF) + 1.0) / 2.0**24
-            a = _slot_angle(w, s)
+            a = w[s] >> 23
             rad = math.sqrt(-2.0 * math.log(u1))
-            want = (rad * math.cos(2 * math.pi * a / 4096), rad * math.sin(2 * math.pi * a / 4096))
+            want = (rad * math.cos(2 * math.pi * a / 512), rad * math.sin(2 * math.pi * a / 512))
             got = orc.heston_normals(seed, p, j)
             assert got == pytest.approx(want, rel=1e-13, abs=1e-13), (p, j)
     # the largest radius: m = 0 -> u1 = 2^-24, sqrt(48 ln 2) = 5.768
     assert math.sqrt(-2 * math.log(2.0**-24)) == pytest.approx(math.sqrt(48 * math.log(2)), rel=1e-15)
+
+
+def test_quad_angles_match_gaussian_moments():
+    """512 equally spaced angles give E[cos^n] and E[cos^n sin^k] of a uniform
+    angle exactly for n + k < 512 (the sums of e^{i q theta} over the grid
+    vanish for 0 < |q| < 512): the stream's pairs have the Gaussian moments
+    of their radius through high order.  A grid of 4 angles would not (its
+    fourth moment of cos is 1/2 instead of 3/8); this is the property the
+    table size is chosen for, checked here numerically."""
+    th = 2 * np.pi * np.arange(512) / 512
+    c, s = np.cos(th), np.sin(th)
+    # E[cos^2] = 1/2, E[cos^4] = 3/8, E[cos^2 sin^2] = 1/8, E[cos^6] = 5/16,
+    # odd moments 0 (double factorial over 2^n for even n)
+    assert np.mean(c**2) == pytest.approx(0.5, abs=1e-15)
+    assert np.mean(c**4) == pytest.approx(3 / 8, abs=1e-15)
+    assert np.mean(c**2 * s**2) == pytest.approx(1 / 8, abs=1e-15)
+    assert np.mean(c**6) == pytest.approx(5 / 16, abs=1e-15)
+    for n in (1, 3, 5, 101, 255):
+        assert abs(np.mean(c**n)) < 1e-13 and abs(np.mean(c**(n - 1) * s)) < 1e-13
+    # a coarse grid fails the fourth moment: the check can see a wrong size
+    th4 = 2 * np.pi * np.arange(4) / 4
+    assert abs(np.mean(np.cos(th4) ** 4) - 3 / 8) > 0.1
 
 
 def test_heston_stream_distribution(orc):
@@ -192,11 +226,11 @@ def test_heston_stream_distribution(orc):
         cdf = 0.5 * np.array([math.erfc(-x / math.sqrt(2)) for x in zs])
         ks = np.max(np.abs(cdf - np.arange(1, zs.size + 1) / zs.size))
         assert ks < 1.63 / math.sqrt(zs.size) * 1.5
-    # z1 and z2 uncorrelated; no correlation between the three slots of a
-    # block (they share words only through disjoint bits) or across blocks
+    # z1 and z2 uncorrelated; no correlation between the four slots of a
+    # block (one word each) or across blocks
     assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 5 / math.sqrt(n)
     zz = z[:, 0].reshape(n_paths, n_steps)
-    for a, b in ((0, 1), (1, 2), (0, 2), (2, 3), (0, 3)):
+    for a, b in ((0, 1), (1, 2), (0, 2), (2, 3), (0, 3), (3, 4), (0, 4)):
         assert abs(np.corrcoef(zz[:, a], zz[:, b])[0, 1]) < 5 / math.sqrt(n_paths)
         assert abs(np.corrcoef(zz[:, a] ** 2, zz[:, b] ** 2)[0, 1]) < 5 / math.sqrt(n_paths)
     assert np.abs(z).max() <= math.sqrt(48 * math.log(2)) + 1e-12
@@ -204,20 +238,20 @@ def test_heston_stream_distribution(orc):
 
 def test_gbm_stream_is_heston_pairs(orc):
     """GBM / local-vol stepwise step j: cosine (even j) or sine (odd j) of the
-    Heston pair of step j/2, so six normals per Philox block (p, j/6, 0, 0)."""
+    Heston pair of step j/2, so eight normals per Philox block (p, j/8, 0, 0)."""
     seed = 0x0BAD_5EED_1234_5678
     for p in (0, 3, 2**31 + 7):
-        for j in range(14):
+        for j in range(18):
             want = orc.heston_normals(seed, p, j // 2)[j % 2]
             assert orc.lib.orc_philox_gbm_normal(seed, p, j) == want, (p, j)
-    # steps 0-5 come from block (p, 0) and step 6 from block (p, 1): restated
+    # steps 0-7 come from block (p, 0) and step 8 from block (p, 1): restated
     # from the Philox words for slot 0 of block 1
     w = orc.philox([5, 1, 0, 0], [seed & 0xFFFFFFFF, seed >> 32])
     u1 = (2.0 * (w[0] & 0x7FFFFF) + 1.0) / 2.0**24
-    ang = 2 * math.pi * ((w[3] >> 3) & 0xFFF) / 4096
+    ang = 2 * math.pi * (w[0] >> 23) / 512
     rad = math.sqrt(-2.0 * math.log(u1))
-    assert orc.lib.orc_philox_gbm_normal(seed, 5, 6) == pytest.approx(rad * math.cos(ang), rel=1e-13, abs=1e-13)
-    assert orc.lib.orc_philox_gbm_normal(seed, 5, 7) == pytest.approx(rad * math.sin(ang), rel=1e-13, abs=1e-13)
+    assert orc.lib.orc_philox_gbm_normal(seed, 5, 8) == pytest.approx(rad * math.cos(ang), rel=1e-13, abs=1e-13)
+    assert orc.lib.orc_philox_gbm_normal(seed, 5, 9) == pytest.approx(rad * math.sin(ang), rel=1e-13, abs=1e-13)
 
 
 def test_terminal_stream_exact_mapping(orc):
@@ -272,7 +306,7 @@ def test_gbm_stream_distribution(orc):
     assert np.max(np.abs(cdf - np.arange(1, zs.size + 1) / zs.size)) < 1.63 / math.sqrt(zs.size) * 1.5
     # consecutive steps (the two halves of one pair, and pairs of one block)
     # and steps of different blocks are uncorrelated, also in their squares
-    for a, b in ((0, 1), (1, 2), (0, 2), (4, 5), (5, 6), (0, 6)):
+    for a, b in ((0, 1), (1, 2), (0, 2), (6, 7), (7, 8), (0, 8)):
         assert abs(np.corrcoef(z[:, a], z[:, b])[0, 1]) < 5 / math.sqrt(n_paths)
         assert abs(np.corrcoef(z[:, a] ** 2, z[:, b] ** 2)[0, 1]) < 5 / math.sqrt(n_paths)
 
