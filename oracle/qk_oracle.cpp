@@ -138,32 +138,7 @@ double philox_path_normal(uint64_t seed, uint32_t p, uint32_t d) {
     return z[d & 3u];
 }
 
-// Three-pair layout of one Philox block (the terminal stream; counter in[],
-// key = seed): slot s takes u1 = (2m + 1) 2^-24 with m the low 23 bits of
-// word s, and a 12-bit
-// angle index a: bits 3-14 of w3 (slot 0); bits 19-23 of w3 below bits 24-30
-// of w0 (slot 1); bits 27-31 of w1 below bits 24-30 of w2 (slot 2).
-// (z1, z2) = sqrt(-2 ln u1) (cos 2 pi a / 4096, sin 2 pi a / 4096) in FP64
-// (the GPU reads FP32 roundings of the same cosine and sine from a table).
-uint32_t slot_angle_index(const uint32_t w[4], uint32_t s) {
-    if (s == 0) return (w[3] >> 3) & 0xFFFu;
-    if (s == 1) return ((w[3] >> 19) & 0x1Fu) | ((w[0] >> 24) & 0x7Fu) << 5;
-    return ((w[1] >> 27) & 0x1Fu) | ((w[2] >> 24) & 0x7Fu) << 5;
-}
-
-void philox_slot_pair(const uint32_t in[4], uint64_t seed, uint32_t s, double* z1, double* z2) {
-    const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
-    uint32_t w[4];
-    philox10(in, key, w);
-    const double u1 = (2.0 * double(w[s] & 0x007FFFFFu) + 1.0) * 0x1p-24;
-    const double f = double(slot_angle_index(w, s)) * (1.0 / 4096.0);
-    const double rad = std::sqrt(-2.0 * std::log(u1));
-    const double ang = 6.283185307179586476925286766559 * f;
-    *z1 = rad * std::cos(ang);
-    *z2 = rad * std::sin(ang);
-}
-
-// Four-pair layout of one Philox block (the stepwise streams): slot s is word
+// Four-pair layout of one Philox block (every production stream): slot s is word
 // s alone, u1 = (2m + 1) 2^-24 with m its low 23 bits and a 9-bit angle
 // index a = its top 9 bits; (z1, z2) = sqrt(-2 ln u1) (cos 2 pi a / 512,
 // sin 2 pi a / 512) in FP64 (the GPU reads FP32 roundings of the same
@@ -197,13 +172,13 @@ double philox_gbm_normal(uint64_t seed, uint32_t p, uint32_t j) {
     return (j & 1u) ? z2 : z1;
 }
 
-// Terminal estimators: draw k is component k%6 of block (k/6, 0, 0, 1) in
-// the three-pair layout: slot (k%6)/2, the cosine for even k, the sine for
+// Terminal estimators: draw k is component k%8 of block (k/8, 0, 0, 1) in
+// the four-pair layout: slot (k%8)/2, the cosine for even k, the sine for
 // odd k.
 double philox_terminal_normal(uint64_t seed, uint32_t k) {
-    const uint32_t in[4] = {k / 6u, 0u, 0u, 1u};
+    const uint32_t in[4] = {k / 8u, 0u, 0u, 1u};
     double z1, z2;
-    philox_slot_pair(in, seed, (k % 6u) / 2u, &z1, &z2);
+    philox_quad_pair(in, seed, (k % 8u) / 2u, &z1, &z2);
     return (k & 1u) ? z2 : z1;
 }
 

@@ -48,11 +48,11 @@ QK_HD bool method_is_terminal(int32_t m) {
     return m == M_STANDARD || m == M_ANTITHETIC || m == M_IMPORTANCE || m == M_PATHWISE ||
            m == M_LR || m == M_CONTROL || m == M_STRATIFIED || m == M_SOBOL || m == M_HALTON;
 }
-// Units per lane group of a terminal estimator: six for those that draw
-// normals (one Philox block gives three Box-Muller pairs), four for the
+// Units per lane group of a terminal estimator: eight for those that draw
+// normals (one Philox block gives four (radius, angle) pairs), four for the
 // uniform-driven ones (stratified, Sobol, Halton).
 QK_HD int32_t term_group_units(int32_t m) {
-    return (m == M_STRATIFIED || m == M_SOBOL || m == M_HALTON) ? 4 : 6;
+    return (m == M_STRATIFIED || m == M_SOBOL || m == M_HALTON) ? 4 : 8;
 }
 // Deterministic point sets: no stream, and no sampling-error estimate.
 QK_HD bool method_is_qmc(int32_t m) { return m == M_SOBOL || m == M_HALTON; }

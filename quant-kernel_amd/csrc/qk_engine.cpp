@@ -116,10 +116,11 @@ void init_device() {
     const int per_cu = std::max(1, env_int("QK_GRID_PER_CU", 8));
     d->shape_step.lane_units = std::max(1, env_int("QK_LANE_PATHS", 8));
     d->shape_step.grid = d->cus * per_cu;
-    // 24 groups (144 antithetic pairs) per lane: an autotuner pick for
-    // 4096 x 100k-path options, where the per-tile fetch and lookup
-    // otherwise weigh on tiles of a few microseconds
-    d->shape_term.lane_units = std::max(1, env_int("QK_LANE_GROUPS", 24));
+    // 25 groups (200 antithetic pairs) per lane: the autotuner's pick for
+    // 4096 x 100k-path options (one 51,200-pair tile covers an option's
+    // 50,000 pairs), where the per-tile fetch and lookup otherwise weigh on
+    // tiles of a few microseconds
+    d->shape_term.lane_units = std::max(1, env_int("QK_LANE_GROUPS", 25));
     d->shape_term.grid = d->cus * per_cu;
     if (env_int("QK_AUTOTUNE", 0)) apply_cached_shapes(d);
     g_dev = d;
@@ -1066,7 +1067,7 @@ int32_t autotune(int32_t family, int32_t* best_lane, int32_t* best_grid, double*
     // lane counts between the powers of two let a tile divide an option's
     // units nearly evenly (50,000 pairs: 7 tiles of 7168 waste 0.3%, where
     // 7 tiles of 8192 waste 14%)
-    const int lanes[] = {2, 3, 4, 5, 6, 7, 8, 12, 16, 24, 32};
+    const int lanes[] = {2, 3, 4, 5, 6, 7, 8, 12, 16, 20, 24, 25, 32};
     const int grids[] = {4, 8, 16};
     constexpr int kLanes = sizeof(lanes) / sizeof(lanes[0]);
     for (int li = 0; rc == QK_OK && li < kLanes; ++li)

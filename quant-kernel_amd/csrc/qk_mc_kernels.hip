@@ -78,7 +78,7 @@ struct Draws {
 struct PhiloxSrc {
     uint32_t k0, k1;
     uint32_t tag = 0;               // fourth counter word of path blocks (the MLMC level)
-    const float2* trig = nullptr;   // LDS angle table of the three-pair (terminal) layout
+    const float2* trig = nullptr;   // LDS angle table of the four-pair layout (terminal draws)
     __device__ __forceinline__ Quad<float> path_block(uint32_t p, uint32_t b, int32_t) const {
         const Normal4 n = philox_normals(p, b, tag, k0, k1);
         return Quad<float>{{n.z0, n.z1, n.z2, n.z3}};
@@ -91,15 +91,15 @@ struct PhiloxSrc {
         u[2] = (double(w.z) + 0.5) * 0x1p-32;
         u[3] = (double(w.w) + 0.5) * 0x1p-32;
     }
-    // normals k = 6b .. 6b+5 of a terminal estimator: the three pairs of
+    // normals k = 8b .. 8b+7 of a terminal estimator: the four pairs of
     // block (b, 0, 0, 1)
-    __device__ __forceinline__ Draws<float, 6> term_normals6(uint32_t b) const {
-        const Normal6 n = philox_normals6(philox4x32_10(b, 0u, 1u, k0, k1), trig);
-        return Draws<float, 6>{{n.z[0], n.z[1], n.z[2], n.z[3], n.z[4], n.z[5]}};
+    __device__ __forceinline__ Draws<float, 8> term_normals8(uint32_t b) const {
+        const Normal8 n = philox_normals8(philox4x32_10(b, 0u, 1u, k0, k1), trig);
+        return Draws<float, 8>{{n.z[0], n.z[1], n.z[2], n.z[3], n.z[4], n.z[5], n.z[6], n.z[7]}};
     }
     __device__ __forceinline__ float term_draw(uint32_t k) const {
-        const Draws<float, 6> q = term_normals6(k / 6u);
-        return q.z[k % 6u];
+        const Draws<float, 8> q = term_normals8(k / 8u);
+        return q.z[k % 8u];
     }
 };
 
@@ -134,8 +134,8 @@ struct StreamSrc {
 };
 
 // ----------------------------------------------------------------------------
-// angle table of the three-pair Philox layout: entry a = (cos, sin) of
-// 2 pi a / 4096, FP64 values rounded to FP32 on the host (upload_trig_table)
+// angle table of the four-pair Philox layout: entry a = (cos, sin) of
+// 2 pi a / 512, FP64 values rounded to FP32 on the host (upload_trig_table)
 // ----------------------------------------------------------------------------
 __device__ __align__(16) float2 g_trig[kTrigEntries];
 
@@ -569,8 +569,8 @@ __device__ __forceinline__ void retable(float2* s_trig, TrigTableState* st, bool
     const int32_t want = heston ? 1 : 0;
     if (st->kind == want && (!heston || (st->A == A && st->B == B))) return;
     __syncthreads();  // every thread has read st and finished with the old contents
-    for (int i = threadIdx.x; i < kQuadAngles; i += kBlock) {
-        const float2 c = g_trig[i * (kTrigEntries / kQuadAngles)];  // angle 2 pi i / 512
+    for (int i = threadIdx.x; i < kTrigEntries; i += kBlock) {
+        const float2 c = g_trig[i];  // angle 2 pi i / 512
         s_trig[i] = heston ? make_float2(c.x, __builtin_fmaf(B, c.y, A * c.x)) : c;
     }
     if (threadIdx.x == 0) {
@@ -781,7 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_sim_stepwise(const OptionRec* __rest
                                                          int32_t lane_paths) {
     __shared__ int32_t s_tile;
     __shared__ double s_red[kBlock / 64][5];
-    __shared__ __align__(16) float2 s_trig[MODE == SIM_PHILOX_F32 ? kQuadAngles : 1];
+    __shared__ __align__(16) float2 s_trig[MODE == SIM_PHILOX_F32 ? kTrigEntries : 1];
     __shared__ TrigTableState s_tts;
     const float2* trig = nullptr;
     if constexpr (MODE == SIM_PHILOX_F32) {
@@ -1029,13 +1029,13 @@ __device__ __forceinline__ float term_unit_f32(const TermC<float>& k, float z) {
 }
 
 // One tile's units: lane handles groups g0 + lane + j*kBlock of GU units:
-// six (one Philox block of three Box-Muller pairs) for the estimators that
-// draw normals, four for the uniform-driven ones.  x1, x2, xy accumulate the
-// control (TK_CV only).
+// eight (one Philox block of four (radius, angle) pairs) for the estimators
+// that draw normals, four for the uniform-driven ones.  x1, x2, xy
+// accumulate the control (TK_CV only).
 //
-// FP32 paths (TK_PAIR, TK_LR, TK_IS, TK_PW) sum each group's four centred
+// FP32 paths (TK_PAIR, TK_LR, TK_IS, TK_PW) sum each group's eight centred
 // units and their squares in FP32 and add the two group sums to the lane's
-// FP64 sums: two FP64 additions per four units instead of five FP64
+// FP64 sums: two FP64 additions per eight units instead of five FP64
 // operations per unit.  The centring uses float(shift); the lane adds
 // count * (float(shift) - shift) at the end, so s1 stays the sum of
 // unit - shift.  The FP32 group sums round at 2^-24 of a few units, far below
@@ -1046,7 +1046,7 @@ __device__ __forceinline__ void term_tile(const TermC<R>& k, const PS& ph, const
                                           double shift, double& s1, double& s2, double& x1,
                                           double& x2, double& xy) {
     constexpr bool from_u = KIND == TK_STRAT || KIND == TK_SOBOL || KIND == TK_HALTON;
-    constexpr int GU = from_u ? 4 : 6;  // units per group (term_group_units)
+    constexpr int GU = from_u ? 4 : 8;  // units per group (term_group_units)
     constexpr bool f32_sums = sizeof(R) == 4 && !from_u && KIND != TK_CV;
     const float sh = float(shift);
     int32_t cnt = 0;
@@ -1064,8 +1064,8 @@ __device__ __forceinline__ void term_tile(const TermC<R>& k, const PS& ph, const
                 for (int c = 0; c < 4; ++c) uu[c] = double(q.z[c]);
             }
         } else if constexpr (!from_u) {
-            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_normals6(uint32_t(g));
-            else z = ss.template term_draws<6>(uint32_t(g));
+            if constexpr (MODE == SIM_PHILOX_F32) z = ph.term_normals8(uint32_t(g));
+            else z = ss.template term_draws<8>(uint32_t(g));
         }
         if constexpr (f32_sums) {
             const int32_t left = int32_t(units - g * GU);  // >= 1 here
@@ -1341,10 +1341,10 @@ hipError_t launch_philox_probe(hipStream_t st, uint64_t seed, uint32_t c0, uint3
 // host launchers
 // ----------------------------------------------------------------------------
 hipError_t upload_trig_table() {
-    // (cos, sin) of 2 pi a / 4096: a quarter turn in FP64 (the C library's
-    // cos / sin of an exact multiple of 2 pi / 4096, rounded once to FP32),
+    // (cos, sin) of 2 pi a / 512: a quarter turn in FP64 (the C library's
+    // cos / sin of an exact multiple of 2 pi / 512, rounded once to FP32),
     // the other three quarters by exact symmetry, so sign and zero patterns
-    // are exact (entry 1024 is (0, 1)).
+    // are exact (entry 128 is (0, 1)).
     static float2 host[kTrigEntries];
     constexpr int Q = kTrigEntries / 4;
     const double step = 6.283185307179586476925286766559 / kTrigEntries;

@@ -105,27 +105,17 @@ struct Normal4 {
     float z0, z1, z2, z3;
 };
 
-// Three-pair layout helpers.  u23: (2m + 1) 2^-24 from the low 23 bits m of
-// w, exact in FP32 and in (0, 1).
+// Four-pair layout of the production streams: slot s of a Philox block is
+// word s alone.  u1 = (2m + 1) 2^-24 from its low 23 bits m (u23: exact in
+// FP32 and in (0, 1)), and theta = 2 pi a / 512 from its top 9 bits a, read
+// as (cos, sin) from a 512-entry table; quad_offset is the entry's byte
+// offset (8a).
 __device__ __forceinline__ float u23(uint32_t w) {
     return __builtin_bit_cast(float, (w & 0x007FFFFFu) | 0x3F800000u) - 0x1.fffffep-1f;
 }
 
-constexpr int kTrigEntries = 4096;  // angle table: (cos, sin) of 2 pi a / 4096
-
-// Byte offsets into the (cos, sin) table of the three slots' angles: the
-// 12-bit index sits at bits 3-14 of a word (slot 0: w3 itself; slots 1 and 2:
-// two top bytes joined by one v_perm_b32), so one and-mask gives 8 x index.
-struct TrigOff {
-    uint32_t o[3];
-};
-__device__ __forceinline__ TrigOff trig_offsets(const U4& w) {
-    TrigOff t;
-    t.o[0] = w.w & 0x7FF8u;
-    t.o[1] = __builtin_amdgcn_perm(w.x, w.w, 0x0C0C0702u) & 0x7FF8u;  // {w3.byte2, w0.byte3}
-    t.o[2] = __builtin_amdgcn_perm(w.z, w.y, 0x0C0C0703u) & 0x7FF8u;  // {w1.byte3, w2.byte3}
-    return t;
-}
+constexpr int kTrigEntries = 512;  // angle table: (cos, sin) of 2 pi a / 512
+__device__ __forceinline__ uint32_t quad_offset(uint32_t w) { return (w >> 20) & 0xFF8u; }
 __device__ __forceinline__ float2 trig_at(const float2* trig, uint32_t byte_off) {
     return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(trig) + byte_off);
 }
@@ -139,25 +129,20 @@ __device__ __forceinline__ void stage_trig_table(const float2* __restrict__ src,
     __syncthreads();
 }
 
-// Four-pair layout of the stepwise (path) streams: slot s of a block is
-// word s alone, u1 = (2m + 1) 2^-24 from its low 23 bits m and a 9-bit angle
-// index from its top 9 bits, looked up in a 512-entry table (entry a = entry
-// 8a of the three-pair table).  quad_offset gives the byte offset (8 x index).
-constexpr int kQuadAngles = 512;
-__device__ __forceinline__ uint32_t quad_offset(uint32_t w) { return (w >> 20) & 0xFF8u; }
-
-struct Normal6 {
-    float z[6];
+// The eight normals of one block: (r_s cos theta_s, r_s sin theta_s) of
+// slots s = 0 .. 3, r_s = sqrt(-2 ln u1_s).
+struct Normal8 {
+    float z[8];
 };
 
-__device__ __forceinline__ Normal6 philox_normals6(const U4& w, const float2* trig) {
-    const TrigOff t = trig_offsets(w);
-    const float u[3] = {u23(w.x), u23(w.y), u23(w.z)};
-    Normal6 n;
+__device__ __forceinline__ Normal8 philox_normals8(const U4& w, const float2* trig) {
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    Normal8 n;
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
-        const float2 cs = trig_at(trig, t.o[s]);
-        const float rad = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u[s]) * -1.3862943611198906f);
+    for (int s = 0; s < 4; ++s) {
+        const float2 cs = trig_at(trig, quad_offset(ws[s]));
+        const float rad =
+            __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u23(ws[s])) * -1.3862943611198906f);
         n.z[2 * s] = rad * cs.x;
         n.z[2 * s + 1] = rad * cs.y;
     }

@@ -131,35 +131,6 @@ def test_box_muller_exact_mapping(orc):
     assert z0.value == pytest.approx(math.sqrt(2 * 33 * math.log(2)), rel=1e-14) and z1.value == 0.0
 
 
-def _slot_angle(w, s):
-    """12-bit angle index of slot s of a three-pair block, from the words:
-    bits 3-14 of w3 (slot 0); bits 19-23 of w3 below bits 24-30 of w0
-    (slot 1); bits 27-31 of w1 below bits 24-30 of w2 (slot 2)."""
-    if s == 0:
-        return (w[3] >> 3) & 0xFFF
-    lo, hi = (w[3] >> 19, w[0]) if s == 1 else (w[1] >> 27, w[2])
-    return (lo & 0x1F) | ((hi >> 24) & 0x7F) << 5
-
-
-def test_slot_bits_are_disjoint():
-    """The three radii (low 23 bits of w0, w1, w2) and the three 12-bit angle
-    indices use disjoint bits of the block: setting one bit at a time moves
-    at most one field."""
-    def fields(w):
-        return [w[0] & 0x7FFFFF, w[1] & 0x7FFFFF, w[2] & 0x7FFFFF] + [_slot_angle(w, s) for s in range(3)]
-    used = 0
-    for word in range(4):
-        for bit in range(32):
-            w = [0, 0, 0, 0]
-            w[word] = 1 << bit
-            moved = [i for i, f in enumerate(fields(w)) if f]
-            assert len(moved) <= 1, (word, bit, moved)
-            used += len(moved)
-    assert used == 3 * 23 + 3 * 12
-    # every angle index is reachable: all-ones words give 4095 for each slot
-    assert [_slot_angle([0xFFFFFFFF] * 4, s) for s in range(3)] == [4095] * 3
-
-
 def test_quad_slot_bits():
     """Four-pair layout: slot s is word s alone, the radius from its low 23
     bits and the 9-bit angle index from its top 9 bits; the two fields of a
@@ -255,18 +226,18 @@ def test_gbm_stream_is_heston_pairs(orc):
 
 
 def test_terminal_stream_exact_mapping(orc):
-    """Draw k of a terminal estimator: component k%6 of Philox block
-    (k/6, 0, 0, 1) in the three-pair layout, i.e. the cosine (even k) or sine
-    (odd k) of slot (k%6)/2.  Restated from the words with numpy."""
+    """Draw k of a terminal estimator: component k%8 of Philox block
+    (k/8, 0, 0, 1) in the four-pair layout, i.e. the cosine (even k) or sine
+    (odd k) of word (k%8)/2.  Restated from the words."""
     seed = 0x5EED_0F_7E57_C0DE
     key = [seed & 0xFFFFFFFF, seed >> 32]
-    for k in list(range(18)) + [6 * 12345 + 5, 6 * (2**31 + 3) + 2]:
+    for k in list(range(18)) + [8 * 12345 + 5, 8 * (2**28 + 3) + 2]:
         k &= 0xFFFFFFFF
-        w = orc.philox([k // 6, 0, 0, 1], key)
-        s = (k % 6) // 2
+        w = orc.philox([k // 8, 0, 0, 1], key)
+        s = (k % 8) // 2
         u1 = (2.0 * (w[s] & 0x7FFFFF) + 1.0) / 2.0**24
         rad = math.sqrt(-2.0 * math.log(u1))
-        ang = 2 * math.pi * _slot_angle(w, s) / 4096
+        ang = 2 * math.pi * (w[s] >> 23) / 512
         want = rad * (math.sin(ang) if k % 2 else math.cos(ang))
         assert orc.lib.orc_philox_terminal_normal(seed, k) == pytest.approx(want, rel=1e-13, abs=1e-13), k
     # the terminal stream (fourth counter word 1) is not the Heston stream of
@@ -284,10 +255,10 @@ def test_terminal_stream_distribution(orc):
     zs = np.sort(z)
     cdf = 0.5 * np.array([math.erfc(-x / math.sqrt(2)) for x in zs])
     assert np.max(np.abs(cdf - np.arange(1, n + 1) / n)) < 1.63 / math.sqrt(n) * 1.5
-    # the six components of a block are uncorrelated, in value and in square
-    b = z.reshape(-1, 6)
+    # the eight components of a block are uncorrelated, in value and in square
+    b = z.reshape(-1, 8)
     m = b.shape[0]
-    for a, c in ((0, 1), (0, 2), (1, 3), (2, 4), (4, 5), (0, 5)):
+    for a, c in ((0, 1), (0, 2), (1, 3), (2, 4), (4, 5), (0, 5), (6, 7), (1, 7)):
         assert abs(np.corrcoef(b[:, a], b[:, c])[0, 1]) < 5 / math.sqrt(m)
         assert abs(np.corrcoef(b[:, a] ** 2, b[:, c] ** 2)[0, 1]) < 5 / math.sqrt(m)
     assert np.abs(z).max() <= math.sqrt(48 * math.log(2)) + 1e-12
