@@ -181,6 +181,12 @@ PipeBufs context_bufs(Device* d) {
                     d->scan.p, static_cast<Partial*>(d->partials.p)};
 }
 
+// A terminal batch of one method that the pair-only kernel covers.
+bool pair_only_batch(const BatchIn& in, int32_t family) {
+    return family == FAM_TERMINAL && !in.method &&
+           (in.uniform_method == M_STANDARD || in.uniform_method == M_ANTITHETIC);
+}
+
 int32_t launch_pipeline(Device* d, const PipeBufs& bufs, const BatchIn& in, int32_t n,
                         const int32_t* n_dev, const int64_t* stream_off, int32_t family,
                         int32_t mode, const void* stream, double* out_price, double* out_se,
@@ -200,7 +206,8 @@ int32_t launch_pipeline(Device* d, const PipeBufs& bufs, const BatchIn& in, int3
                           recs, ntiles));
     QK_HIP(launch_scan(d->stream, n, n_dev, ntiles, recs, tstart, ctr, bufs.scan, capacity));
     if (timed) QK_HIP(hipEventRecord(d->ev[1], d->stream));
-    QK_HIP(launch_simulate(d->stream, family, mode, shp, recs, tstart, n_dev, ctr, parts, stream));
+    QK_HIP(launch_simulate(d->stream, family, mode, shp, recs, tstart, n_dev, ctr, parts, stream,
+                           pair_only_batch(in, family)));
     if (timed) QK_HIP(hipEventRecord(d->ev[2], d->stream));
     QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, ctr, out_price, out_se, host_flags));
     if (timed) QK_HIP(hipEventRecord(d->ev[3], d->stream));
@@ -620,7 +627,7 @@ int32_t mc_device_locked(Device* d, const BatchIn& in, int32_t n, const int32_t*
     Partial* parts = static_cast<Partial*>(d->partials.p);
     QK_HIP(hipEventRecord(d->ev[1], d->stream));
     QK_HIP(launch_simulate(d->stream, family, SIM_PHILOX_F32, shp, recs, tstart, n_dev, ctr, parts,
-                           nullptr));
+                           nullptr, pair_only_batch(in, family)));
     QK_HIP(hipEventRecord(d->ev[2], d->stream));
     QK_HIP(launch_finalize(d->stream, n_dev, recs, parts, ctr, out_price_dev, out_se_dev, nullptr));
     QK_HIP(hipEventRecord(d->ev[3], d->stream));

@@ -55,10 +55,12 @@ hipError_t launch_prepare(hipStream_t st, const BatchIn& in, const int32_t* n_de
 hipError_t launch_scan(hipStream_t st, int32_t n, const int32_t* n_dev, const int32_t* ntiles,
                        OptionRec* recs, int32_t* tile_start, int32_t* counters, void* scratch,
                        int32_t capacity);
+// pair_only: a terminal batch whose every option is standard MC or
+// antithetic variates (a kernel without the other estimators' branches).
 hipError_t launch_simulate(hipStream_t st, int32_t family, int32_t mode, const SimShape& shape,
                            const OptionRec* recs, const int32_t* tile_start,
                            const int32_t* n_dev, int32_t* counters, Partial* partials,
-                           const void* stream);
+                           const void* stream, bool pair_only);
 // host_flags (nullable): device address of pinned host memory that receives
 // counters[0..3]
 hipError_t launch_finalize(hipStream_t st, const int32_t* n_dev, const OptionRec* recs,
