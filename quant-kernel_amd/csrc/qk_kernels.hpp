@@ -67,7 +67,7 @@ hipError_t launch_finalize(hipStream_t st, const int32_t* n_dev, const OptionRec
                            const Partial* partials, const int32_t* counters, double* out_price,
                            double* out_se, int32_t* host_flags);
 
-// Writes the angle table of the three-pair Philox layout into device memory
+// Writes the angle table of the four-pair Philox layout into device memory
 // (the current device); called once per device at set-up.
 hipError_t upload_trig_table();
 

@@ -11,21 +11,19 @@
 //   f  = (w_odd & 0x7FFFFF) / 2^23                   (exact, in [0, 1))
 //   (z_even, z_odd) = sqrt(-2 ln u1) * (cos 2 pi f, sin 2 pi f)
 //
-// Six normals per block (philox_normals6): three pairs, slot s taking
+// Eight normals per block (philox_normals8), the four-pair layout: slot s
+// is word w_s alone,
 //   u1 = (2m + 1) 2^-24, m = the low 23 bits of w_s  (exact in FP32)
-// and a 12-bit angle index a_s, theta_s = 2 pi a_s / 4096, from bits that no
-// slot's m uses:
-//   slot 0: bits 3-14 of w3;
-//   slot 1: bits 19-23 of w3 (low) and 24-30 of w0 (high);
-//   slot 2: bits 27-31 of w1 (low) and 24-30 of w2 (high);
+//   theta_s = 2 pi a_s / 512, a_s = the top 9 bits of w_s,
 // (z_2s, z_2s+1) = sqrt(-2 ln u1) (cos theta_s, sin theta_s), with the
-// cosine and sine read from a 4096-entry table (trig_table_entry) that each
-// workgroup stages in LDS.  The radius is capped at sqrt(48 ln 2) = 5.77
-// sigma; 12-bit angles reproduce the trigonometric moments of a uniform angle
-// up to order 4095.  Draw k of a terminal estimator is component k%6 of block
-// (k/6, 0, 0, 1) (cosine of slot (k%6)/2 for even k, sine for odd k).  Heston
-// paths take step j from slot j%3 of block (p, j/3, 0, 0) (both components),
-// stepwise GBM paths step j from component j%2 of the pair of Heston step j/2.
+// cosine and sine read from a 512-entry table that each workgroup stages in
+// LDS.  The radius is capped at sqrt(48 ln 2) = 5.77 sigma; 512 equally
+// spaced angles give E[cos^n sin^k] of a uniform angle exactly for
+// n + k < 512.  Draw k of a terminal estimator is component k%8 of block
+// (k/8, 0, 0, 1) (cosine of slot (k%8)/2 for even k, sine for odd k).
+// Heston paths take step j from slot j%4 of block (p, j/4, 0, 0) (both
+// components), stepwise GBM paths step j from component j%2 of the pair of
+// Heston step j/2.
 #pragma once
 
 #include <hip/hip_runtime.h>
