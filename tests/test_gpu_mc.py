@@ -459,8 +459,9 @@ def test_device_entry_points_equal_host_paths_and_replay(qk):
 
 def test_device_partials_grow_beyond_capacity(qk):
     """A device batch needing more tiles than the partials buffer holds runs
-    again after growing it: 80 options x (2^31 - 1) paths = 2.3M tiles at the
-    default shape (24 six-pair groups per lane), against 2^20 at first."""
+    again after growing it: 80 options x (2^31 - 1) paths = 1.7M tiles at the
+    default shape (25 groups of eight antithetic pairs per lane), against
+    2^20 at first."""
     from hip_mem import DevArray
     n = 80
     d = wl.config0(n, paths=1)
