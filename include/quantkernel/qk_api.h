@@ -423,6 +423,15 @@ QK_EXPORT int32_t qk_mc_validation_stream(uint64_t seed, int64_t count, double* 
 QK_EXPORT int32_t qk_mc_philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3,
                                      int32_t count, uint32_t* out_words, float* out_normals);
 
+/* Self-check of the production normal layout: the eight FP32 normals of
+ * blocks (c0 + i, c1, 0, c3), i < count, keyed by seed, in the four-pair
+ * layout the path and terminal kernels draw from (word s gives a 23-bit
+ * radius and a 9-bit angle read from the device's 512-entry table;
+ * normals 2s and 2s+1 are its cosine and sine components).  No reference
+ * counterpart (an addition of this ABI, like qk_mc_philox_probe). */
+QK_EXPORT int32_t qk_mc_philox_probe8(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3,
+                                      int32_t count, float* out_normals);
+
 /* Launch graphs of the QK_RNG_PHILOX paths (on by default): each distinct
  * batch shape of the host-memory entry points is captured once (H2D copy,
  * prepare, scan, simulate, finalize, D2H copy) and replayed, and likewise each

@@ -1074,6 +1074,18 @@ int32_t qk_mc_philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3,
     return rc;
 }
 
+int32_t qk_mc_philox_probe8(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3, int32_t count,
+                            float* out_normals) {
+    if (!check_ptrs({QK_P(out_normals)})) return QK_ERR_NULL_PTR;
+    if (!check_n(count)) return QK_ERR_BAD_SIZE;
+    std::string err;
+    const int32_t rc = guard(err, [&] {
+        return qk::philox_probe8(seed, c0, c1, c3, count, out_normals, &err);
+    });
+    if (rc != QK_OK) set_msg(err);
+    return rc;
+}
+
 int32_t qk_mc_graphs(int32_t enable, int32_t* out_cached, int64_t* out_launches) {
     std::string err;
     const int32_t rc = guard(err, [&] {

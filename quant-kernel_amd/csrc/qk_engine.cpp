@@ -279,6 +279,22 @@ int32_t philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3, int32
     return QK_OK;
 }
 
+int32_t philox_probe8(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3, int32_t count,
+                      float* normals, std::string* err) {
+    Device* d = get_device(err);
+    if (!d) return QK_ERR_RUNTIME_INIT;
+    std::lock_guard<std::mutex> lock(d->mu);
+    QK_HIP(hipSetDevice(d->dev));
+    const size_t bytes = size_t(count) * 32;
+    QK_HIP(d->out.ensure(bytes));
+    QK_HIP(d->pin_out.ensure(bytes));
+    QK_HIP(launch_philox_probe8(d->stream, seed, c0, c1, c3, count, static_cast<float*>(d->out.p)));
+    QK_HIP(hipMemcpyAsync(d->pin_out.p, d->out.p, bytes, hipMemcpyDeviceToHost, d->stream));
+    QK_HIP(hipStreamSynchronize(d->stream));
+    std::memcpy(normals, d->pin_out.p, bytes);
+    return QK_OK;
+}
+
 bool last_timing(double* ms_total, double* ms_simulate) {
     if (!g_dev || g_dev->ms_total < 0.0) return false;
     if (ms_total) *ms_total = g_dev->ms_total;

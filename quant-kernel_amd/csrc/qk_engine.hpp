@@ -70,6 +70,9 @@ int32_t get_rng_mode();
 void host_normal_stream(uint64_t seed, int64_t count, double* out);
 
 // Device Philox words and FP32 normals of blocks (c0+i, c1, 0, c3), i < count.
+// The eight four-pair normals of each of `count` blocks (c0 + i, c1, 0, c3).
+int32_t philox_probe8(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3, int32_t count,
+                      float* normals, std::string* err);
 int32_t philox_probe(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c3, int32_t count,
                      uint32_t* words, float* normals, std::string* err);
 

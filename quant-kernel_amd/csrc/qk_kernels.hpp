@@ -71,6 +71,8 @@ hipError_t launch_finalize(hipStream_t st, const int32_t* n_dev, const OptionRec
 // (the current device); called once per device at set-up.
 hipError_t upload_trig_table();
 
+hipError_t launch_philox_probe8(hipStream_t st, uint64_t seed, uint32_t c0, uint32_t c1,
+                                uint32_t c3, int32_t count, float* normals);
 hipError_t launch_philox_probe(hipStream_t st, uint64_t seed, uint32_t c0, uint32_t c1,
                                uint32_t c3, int32_t count, uint32_t* words, float* normals);
 

@@ -1337,6 +1337,25 @@ __global__ __launch_bounds__(256) void k_philox_probe(uint32_t k0, uint32_t k1, 
     normals[4 * i + 3] = d;
 }
 
+// The eight normals of the four-pair layout of blocks (c0 + i, c1, 0, c3),
+// read through the device's angle table as the path kernels read them.
+__global__ __launch_bounds__(256) void k_philox_probe8(uint32_t k0, uint32_t k1, uint32_t c0,
+                                                       uint32_t c1, uint32_t c3, int32_t count,
+                                                       float* __restrict__ normals) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const Normal8 n = philox_normals8(philox4x32_10(c0 + uint32_t(i), c1, c3, k0, k1), g_trig);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) normals[8 * i + k] = n.z[k];
+}
+
+hipError_t launch_philox_probe8(hipStream_t st, uint64_t seed, uint32_t c0, uint32_t c1,
+                                uint32_t c3, int32_t count, float* normals) {
+    hipLaunchKernelGGL(k_philox_probe8, dim3((count + 255) / 256), dim3(256), 0, st,
+                       uint32_t(seed), uint32_t(seed >> 32), c0, c1, c3, count, normals);
+    return hipGetLastError();
+}
+
 hipError_t launch_philox_probe(hipStream_t st, uint64_t seed, uint32_t c0, uint32_t c1,
                                uint32_t c3, int32_t count, uint32_t* words, float* normals) {
     hipLaunchKernelGGL(k_philox_probe, dim3((count + 255) / 256), dim3(256), 0, st,

@@ -145,6 +145,7 @@ _SIGNATURES = {
     "qk_mc_last_timing": (_I, [_PD, _PD]),
     "qk_mc_validation_stream": (_I, [_U, ct.c_int64, _PD]),
     "qk_device_info": (_I, [ct.c_char_p, _I, _PI]),
+    "qk_mc_philox_probe8": (_I, [_U, ct.c_uint32, ct.c_uint32, ct.c_uint32, _I, ct.POINTER(ct.c_float)]),
     "qk_mc_philox_probe": (_I, [_U, ct.c_uint32, ct.c_uint32, ct.c_uint32, _I,
                                 ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_float)]),
     "qk_mc_graphs": (_I, [_I, _PI, ct.POINTER(ct.c_int64)]),
@@ -278,6 +279,14 @@ class QuantKernel:
                                                  normals.ctypes.data_as(ct.POINTER(ct.c_float))),
                     "qk_mc_philox_probe")
         return words.reshape(count, 4), normals.reshape(count, 4)
+
+    def philox_probe8(self, seed: int, c0: int, c1: int, c3: int, count: int):
+        """Eight normals per block of the four-pair layout, shape (count, 8)."""
+        normals = np.empty(8 * count, dtype=np.float32)
+        self._check(self._lib.qk_mc_philox_probe8(int(seed), int(c0), int(c1), int(c3), int(count),
+                                                  normals.ctypes.data_as(ct.POINTER(ct.c_float))),
+                    "qk_mc_philox_probe8")
+        return normals.reshape(count, 8)
 
     # ---- Heston surrogate (ABI 2.13) --------------------------------------
     def sur_param_count(self) -> int:

@@ -326,6 +326,30 @@ def test_device_philox_bitwise_and_normals(qk, orc):
                 assert abs(normals[i][j + 1] - b.value) < 2e-6 * max(1.0, abs(b.value))
 
 
+def test_device_four_pair_normals_match_oracle(qk, orc):
+    """The production layout as the device forms it (Philox words, 23-bit
+    radius, 9-bit angle through the device's 512-entry FP32 table) equals
+    the oracle's FP64 restatement to FP32 accuracy: the terminal stream
+    (blocks (k/8, 0, 0, 1)) and the Heston stream (blocks (p, j/4, 0, 0)),
+    for the first, a middle and a large block index."""
+    for seed in (0, 0xFFFFFFFFFFFFFFFF, 0x299F31D0A4093822):
+        z = qk.philox_probe8(seed, 0, 0, 1, 1024)
+        for i in (0, 1, 511, 1023):
+            for j in range(8):
+                want = orc.lib.orc_philox_terminal_normal(seed, 8 * i + j)
+                assert abs(z[i][j] - want) < 2e-6 * max(1.0, abs(want)), (seed, i, j)
+        for p, b in ((0, 0), (77, 5), (2**31 + 3, 63)):
+            z = qk.philox_probe8(seed, p, b, 0, 1)
+            for s in range(4):
+                want = orc.heston_normals(seed, p, 4 * b + s)
+                assert abs(z[0][2 * s] - want[0]) < 2e-6 * max(1.0, abs(want[0])), (seed, p, b, s)
+                assert abs(z[0][2 * s + 1] - want[1]) < 2e-6 * max(1.0, abs(want[1])), (seed, p, b, s)
+    # every (radius, angle) pair stays inside the 5.77 sigma radius cap
+    z = qk.philox_probe8(12345, 0, 0, 0, 65536).reshape(-1, 2)
+    r = np.hypot(z[:, 0].astype(np.float64), z[:, 1].astype(np.float64))
+    assert r.max() <= math.sqrt(48 * math.log(2)) * (1 + 1e-6)
+
+
 def test_graph_replays_match_direct_launches(qk):
     """The host path's captured graph, replayed with other device work in
     between, gives bitwise the results of direct launches."""
